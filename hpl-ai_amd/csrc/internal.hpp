@@ -1,0 +1,75 @@
+// internal.hpp — host-side plumbing shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/hplai.h"
+
+struct hplai_ctx_s {
+    int device = 0;
+    int num_cu = 0;
+    hipStream_t side = nullptr;    // second stream for look-ahead panel work
+    hipEvent_t ev[8] = {};
+    // buffers kept by hplai_solve(keep_factors)
+    float *A = nullptr;
+    int64_t n = 0, lda = 0;
+    double *x = nullptr;
+};
+
+namespace hplai {
+
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+
+#define HPLAI_HIP(call)                                              \
+    do {                                                             \
+        hipError_t _e = (call);                                      \
+        if (_e != hipSuccess) return ::hplai::hip_fail(_e, #call);   \
+    } while (0)
+
+#define HPLAI_CHECK(cond, ...)                                       \
+    do {                                                             \
+        if (!(cond)) {                                               \
+            ::hplai::set_error(__VA_ARGS__);                         \
+            return HPLAI_EINVAL;                                     \
+        }                                                            \
+    } while (0)
+
+inline hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---- kernel launchers (no argument validation; callers validate) ----
+// accumulate != 0: A += generated block (lazy initialisation)
+int launch_gen_matrix(int64_t n, uint64_t seed, int64_t i0, int64_t j0, int64_t m, int64_t k,
+                      const double *d, float *A, int64_t lda, int accumulate, hipStream_t s);
+int launch_gen_diag(int64_t n, uint64_t seed, int64_t i0, int64_t cnt, double *d, hipStream_t s);
+int launch_gen_rhs(int64_t n, uint64_t seed, int64_t i0, int64_t cnt, double *b, hipStream_t s);
+
+int launch_getrf(int n, float *A, int64_t lda, int *info, hipStream_t s);
+int launch_trsm_l(int64_t m, int b, const float *U, int64_t ldu, float *L, int64_t ldl,
+                  float *amax, int tb, hipStream_t s);
+int launch_trsm_u(int b, int64_t n, const float *Lkk, int64_t ldl, float *U, int64_t ldu,
+                  float *amax, int tb, hipStream_t s);
+// amax: per-tile maxima of |src| (nullable for FMT_F16 → exponents 0).
+int launch_quantize(int fmt, int side, int64_t m, int64_t k, int tb, const float *src, int64_t lds,
+                    void *dst, int64_t ldd, const float *amax, int32_t *exps, hipStream_t s);
+int launch_tile_amax(int side, int64_t m, int64_t k, int tb, const float *src, int64_t lds,
+                     float *amax, hipStream_t s);
+int launch_fill_f32(float *p, int64_t cnt, float v, hipStream_t s);
+int launch_schur(int fmt, int64_t m, int64_t n, int nslices, int kslice, const void *Lq,
+                 int64_t ldl, const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
+                 const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, int num_cu,
+                 hipStream_t s);
+
+int launch_otf_gemv(int64_t n, uint64_t seed, const double *d, double alpha, const double *x,
+                    double *y, double *work, hipStream_t s);
+int64_t otf_gemv_work_doubles(int64_t n);
+int launch_trsv_lower(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s);
+int launch_trsv_upper(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s);
+int launch_norm_inf(int64_t n, const double *x, double *out, hipStream_t s);
+int launch_vec_div(int64_t n, const double *num, const double *den, double *out, hipStream_t s);
+int launch_vec_add(int64_t n, const double *x, double *y, hipStream_t s);
+int launch_checksum(int64_t n, const float *LU, int64_t lda, double *out3, hipStream_t s);
+
+}  // namespace hplai

@@ -1,0 +1,334 @@
+// ir.hip — FP64 iterative-refinement kernels on gfx950: the matrix-free
+// residual GEMV (A regenerated from the LCG), the blocked triangular solves
+// with FP32 factors and FP64 vectors, norms, vector updates and the factor
+// checksum.
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace hplai {
+
+// ---------------------------------------------------------------------------
+// otf_gemv (panel_gemv.hpp:57-83, otf_gemv.cpp:5-131).
+// Pass 1: partial[c][i] = sum_{j in chunk c, j != i} (double)(int64)state_ij * x_j
+// (raw integers, like otf_gemv_kernel; the 2^-64-style scale is applied once).
+// Pass 2: y_i += alpha * (scale * sum_c partial[c][i] + d_i x_i), chunks in order.
+// ---------------------------------------------------------------------------
+constexpr int kGemvRows = 256;
+constexpr int kGemvChunk = 2048;
+
+__global__ __launch_bounds__(kGemvRows) void otf_gemv_partial_kernel(
+    int64_t n, uint64_t x1, int64_t row_blocks, LcgCoeff jn, LcgCoeff jn2,
+    const double *__restrict__ x, double *__restrict__ partial)
+{
+    __shared__ double xs[kGemvChunk];
+    const int64_t rb = blockIdx.x % row_blocks;
+    const int64_t cb = blockIdx.x / row_blocks;
+    const int64_t j0 = cb * kGemvChunk;
+    const int64_t j1 = j0 + kGemvChunk < n ? j0 + kGemvChunk : n;
+    for (int64_t t = threadIdx.x; t < j1 - j0; t += kGemvRows) xs[t] = x[j0 + t];
+    __syncthreads();
+    const int64_t i = rb * kGemvRows + threadIdx.x;
+    if (i >= n) return;
+    uint64_t s0 = lcg_jump(x1, static_cast<uint64_t>(i) + static_cast<uint64_t>(n) * static_cast<uint64_t>(j0));
+    uint64_t s1 = jn.a * s0 + jn.c;
+    double acc0 = 0.0, acc1 = 0.0;
+    const int64_t len = j1 - j0;
+    const int64_t idiag = i - j0;   // local column of the diagonal, if in this chunk
+    int64_t t = 0;
+    for (; t + 1 < len; t += 2) {
+        double a0 = static_cast<double>(static_cast<int64_t>(s0));
+        double a1 = static_cast<double>(static_cast<int64_t>(s1));
+        a0 = (t == idiag) ? 0.0 : a0;
+        a1 = (t + 1 == idiag) ? 0.0 : a1;
+        acc0 = fma(a0, xs[t], acc0);
+        acc1 = fma(a1, xs[t + 1], acc1);
+        s0 = jn2.a * s0 + jn2.c;
+        s1 = jn2.a * s1 + jn2.c;
+    }
+    if (t < len) {
+        double a0 = static_cast<double>(static_cast<int64_t>(s0));
+        a0 = (t == idiag) ? 0.0 : a0;
+        acc0 = fma(a0, xs[t], acc0);
+    }
+    partial[cb * n + i] = acc0 + acc1;
+}
+
+__global__ void otf_gemv_reduce_kernel(int64_t n, int64_t nchunks, double alpha,
+                                       const double *__restrict__ d, const double *__restrict__ x,
+                                       const double *__restrict__ partial, double *__restrict__ y)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int64_t c = 0; c < nchunks; ++c) s += partial[c * n + i];
+    const double ax = fma(s, 0x1.fffffffffffffP-65, d[i] * x[i]);
+    y[i] = fma(alpha, ax, y[i]);
+}
+
+int64_t otf_gemv_work_doubles(int64_t n) { return ceil_div(n, kGemvChunk) * n; }
+
+int launch_otf_gemv(int64_t n, uint64_t seed, const double *d, double alpha, const double *x,
+                    double *y, double *work, hipStream_t s)
+{
+    if (n <= 0) return HPLAI_OK;
+    const int64_t row_blocks = ceil_div(n, kGemvRows);
+    const int64_t nchunks = ceil_div(n, kGemvChunk);
+    const LcgCoeff jn = lcg_pow_host(static_cast<uint64_t>(n));
+    const LcgCoeff jn2 = lcg_mul(jn, jn);
+    hipLaunchKernelGGL(otf_gemv_partial_kernel, dim3(static_cast<unsigned>(row_blocks * nchunks)),
+                       dim3(kGemvRows), 0, s, n, lcg_init(seed), row_blocks, jn, jn2, x, work);
+    HPLAI_HIP(hipGetLastError());
+    hipLaunchKernelGGL(otf_gemv_reduce_kernel, dim3(static_cast<unsigned>(ceil_div(n, 256))),
+                       dim3(256), 0, s, n, nchunks, alpha, d, x, work, y);
+    HPLAI_HIP(hipGetLastError());
+    return HPLAI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solves (panel_trsvL / panel_trsvU, panel_trsv.hpp:310-758, with
+// tttrsvL/U, 115-153, and ttgemv, 10-45): FP32 factor entries widened to FP64.
+// Per block of bs rows: (1) one workgroup solves the diagonal block with x in
+// LDS, 32-row sub-blocks solved by one wave with shuffles; (2) a GEMV updates
+// the rest of the vector with the block's column panel.
+// ---------------------------------------------------------------------------
+constexpr int kTrsvMax = 1024;
+
+__global__ __launch_bounds__(256) void trsv_diag_lower_kernel(int bs, const float *__restrict__ Lkk,
+                                                              int64_t lda, double *__restrict__ x)
+{
+    __shared__ double xs[kTrsvMax];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int r = tid; r < bs; r += 256) xs[r] = x[r];
+    __syncthreads();
+    for (int sb = 0; sb < bs; sb += 32) {
+        const int w = bs - sb < 32 ? bs - sb : 32;
+        if (wave == 0) {
+            double xr = lane < w ? xs[sb + lane] : 0.0;
+            for (int c = 0; c < w; ++c) {
+                const double xc = __shfl(xr, c);
+                if (lane > c && lane < w)
+                    xr = xr - xc * static_cast<double>(Lkk[static_cast<int64_t>(sb + c) * lda + sb + lane]);
+            }
+            if (lane < w) xs[sb + lane] = xr;
+        }
+        __syncthreads();
+        for (int r = sb + w + tid; r < bs; r += 256) {
+            double acc = 0.0;
+            for (int c = 0; c < w; ++c)
+                acc = fma(static_cast<double>(Lkk[static_cast<int64_t>(sb + c) * lda + r]), xs[sb + c], acc);
+            xs[r] -= acc;
+        }
+        __syncthreads();
+    }
+    for (int r = tid; r < bs; r += 256) x[r] = xs[r];
+}
+
+__global__ __launch_bounds__(256) void trsv_diag_upper_kernel(int bs, const float *__restrict__ Ukk,
+                                                              int64_t lda, double *__restrict__ x)
+{
+    __shared__ double xs[kTrsvMax];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int r = tid; r < bs; r += 256) xs[r] = x[r];
+    __syncthreads();
+    const int nsb = (bs + 31) / 32;
+    for (int q = nsb - 1; q >= 0; --q) {
+        const int sb = q * 32;
+        const int w = bs - sb < 32 ? bs - sb : 32;
+        if (wave == 0) {
+            double xr = lane < w ? xs[sb + lane] : 0.0;
+            for (int c = w - 1; c >= 0; --c) {
+                if (lane == c)
+                    xr = xr / static_cast<double>(Ukk[static_cast<int64_t>(sb + c) * lda + sb + c]);
+                const double xc = __shfl(xr, c);
+                if (lane < c)
+                    xr = xr - xc * static_cast<double>(Ukk[static_cast<int64_t>(sb + c) * lda + sb + lane]);
+            }
+            if (lane < w) xs[sb + lane] = xr;
+        }
+        __syncthreads();
+        for (int r = tid; r < sb; r += 256) {
+            double acc = 0.0;
+            for (int c = 0; c < w; ++c)
+                acc = fma(static_cast<double>(Ukk[static_cast<int64_t>(sb + c) * lda + r]), xs[sb + c], acc);
+            xs[r] -= acc;
+        }
+        __syncthreads();
+    }
+    for (int r = tid; r < bs; r += 256) x[r] = xs[r];
+}
+
+// y(0:m) -= P(m x w) * xb(0:w), P column-major (ttgemv, panel_trsv.hpp:10-45)
+__global__ __launch_bounds__(256) void trsv_gemv_kernel(int64_t m, int w, const float *__restrict__ P,
+                                                        int64_t lda, const double *__restrict__ xb,
+                                                        double *__restrict__ y)
+{
+    __shared__ double xs[kTrsvMax];
+    for (int c = threadIdx.x; c < w; c += 256) xs[c] = xb[c];
+    __syncthreads();
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (r >= m) return;
+    double acc0 = 0.0, acc1 = 0.0;
+    int c = 0;
+    for (; c + 1 < w; c += 2) {
+        acc0 = fma(static_cast<double>(P[static_cast<int64_t>(c) * lda + r]), xs[c], acc0);
+        acc1 = fma(static_cast<double>(P[static_cast<int64_t>(c + 1) * lda + r]), xs[c + 1], acc1);
+    }
+    if (c < w) acc0 = fma(static_cast<double>(P[static_cast<int64_t>(c) * lda + r]), xs[c], acc0);
+    y[r] -= acc0 + acc1;
+}
+
+int launch_trsv_lower(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s)
+{
+    for (int64_t kb = 0; kb < n; kb += bs) {
+        const int w = n - kb < bs ? static_cast<int>(n - kb) : bs;
+        const float *Lkk = LU + kb * lda + kb;
+        hipLaunchKernelGGL(trsv_diag_lower_kernel, dim3(1), dim3(256), 0, s, w, Lkk, lda, x + kb);
+        HPLAI_HIP(hipGetLastError());
+        const int64_t m = n - kb - w;
+        if (m > 0) {
+            hipLaunchKernelGGL(trsv_gemv_kernel, dim3(static_cast<unsigned>(ceil_div(m, 256))),
+                               dim3(256), 0, s, m, w, Lkk + w, lda, x + kb, x + kb + w);
+            HPLAI_HIP(hipGetLastError());
+        }
+    }
+    return HPLAI_OK;
+}
+
+int launch_trsv_upper(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s)
+{
+    const int64_t nblk = ceil_div(n, bs);
+    for (int64_t q = nblk - 1; q >= 0; --q) {
+        const int64_t kb = q * bs;
+        const int w = n - kb < bs ? static_cast<int>(n - kb) : bs;
+        const float *Ukk = LU + kb * lda + kb;
+        hipLaunchKernelGGL(trsv_diag_upper_kernel, dim3(1), dim3(256), 0, s, w, Ukk, lda, x + kb);
+        HPLAI_HIP(hipGetLastError());
+        if (kb > 0) {
+            hipLaunchKernelGGL(trsv_gemv_kernel, dim3(static_cast<unsigned>(ceil_div(kb, 256))),
+                               dim3(256), 0, s, kb, w, LU + kb * lda, lda, x + kb, x);
+            HPLAI_HIP(hipGetLastError());
+        }
+    }
+    return HPLAI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// norms and vector ops (panel_norm.hpp:8-41, panel.hpp:277-330)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void norm_inf_kernel(int64_t n, const double *__restrict__ x,
+                                                        double *__restrict__ out)
+{
+    __shared__ double red[1024];
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) m = fmax(m, fabs(x[i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+__global__ void vec_div_kernel(int64_t n, const double *num, const double *den, double *out)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = num[i] / den[i];
+}
+
+__global__ void vec_add_kernel(int64_t n, const double *x, double *y)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += x[i];
+}
+
+int launch_norm_inf(int64_t n, const double *x, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(norm_inf_kernel, dim3(1), dim3(1024), 0, s, n, x, out);
+    HPLAI_HIP(hipGetLastError());
+    return HPLAI_OK;
+}
+
+int launch_vec_div(int64_t n, const double *num, const double *den, double *out, hipStream_t s)
+{
+    if (n <= 0) return HPLAI_OK;
+    hipLaunchKernelGGL(vec_div_kernel, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s,
+                       n, num, den, out);
+    HPLAI_HIP(hipGetLastError());
+    return HPLAI_OK;
+}
+
+int launch_vec_add(int64_t n, const double *x, double *y, hipStream_t s)
+{
+    if (n <= 0) return HPLAI_OK;
+    hipLaunchKernelGGL(vec_add_kernel, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s,
+                       n, x, y);
+    HPLAI_HIP(hipGetLastError());
+    return HPLAI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// panel_check fingerprint (panel_check.hpp:9-65): sum |diag|, |strict U|,
+// |strict L| in FP64.  Column sums then a fixed-order final pass, so the
+// result is deterministic.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void checksum_col_kernel(int64_t n, const float *__restrict__ LU,
+                                                           int64_t lda, double *__restrict__ cols)
+{
+    __shared__ double red[3][256];
+    const int64_t j = blockIdx.x;
+    const float *c = LU + j * lda;
+    double su = 0.0, sl = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const double v = fabs(static_cast<double>(c[i]));
+        if (i < j) su += v;
+        else if (i > j) sl += v;
+    }
+    red[0][threadIdx.x] = su;
+    red[1][threadIdx.x] = sl;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + off];
+            red[1][threadIdx.x] += red[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cols[j] = fabs(static_cast<double>(c[j]));
+        cols[n + j] = red[0][0];
+        cols[2 * n + j] = red[1][0];
+    }
+}
+
+__global__ __launch_bounds__(1024) void checksum_final_kernel(int64_t n, const double *__restrict__ cols,
+                                                              double *__restrict__ out)
+{
+    __shared__ double red[1024];
+    for (int q = 0; q < 3; ++q) {
+        double s = 0.0;
+        for (int64_t j = threadIdx.x; j < n; j += 1024) s += cols[q * n + j];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int off = 512; off > 0; off >>= 1) {
+            if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[q] = red[0];
+        __syncthreads();
+    }
+}
+
+int launch_checksum(int64_t n, const float *LU, int64_t lda, double *out3, hipStream_t s)
+{
+    double *cols = nullptr;
+    HPLAI_HIP(hipMallocAsync(reinterpret_cast<void **>(&cols), sizeof(double) * 3 * n, s));
+    hipLaunchKernelGGL(checksum_col_kernel, dim3(static_cast<unsigned>(n)), dim3(256), 0, s, n, LU, lda,
+                       cols);
+    hipLaunchKernelGGL(checksum_final_kernel, dim3(1), dim3(1024), 0, s, n, cols, out3);
+    HPLAI_HIP(hipGetLastError());
+    HPLAI_HIP(hipFreeAsync(cols, s));
+    return HPLAI_OK;
+}
+
+}  // namespace hplai

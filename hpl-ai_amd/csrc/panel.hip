@@ -1,0 +1,436 @@
+// panel.hip — FP32 panel kernels of the blocked no-pivot LU on gfx950:
+// diagonal-block LU, the two panel TRSMs, per-tile |max| and the
+// FP32 -> FP16 / FP8-e4m3 quantizer that feeds the MFMA Schur update.
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace hplai {
+
+// ---------------------------------------------------------------------------
+// atomic max of a non-negative float, ordered as its uint32 bit pattern
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void atomic_max_pos(float *p, float v)
+{
+    atomicMax(reinterpret_cast<unsigned int *>(p), __float_as_uint(v));
+}
+
+// Max of v over the lanes of the wave that share `tile`; one atomic per tile.
+__device__ __forceinline__ void tile_max_reduce(float *amax, int64_t tile, float v, bool active)
+{
+    if (amax == nullptr) return;
+    const int64_t t0 = __shfl(tile, 0);
+    const bool uniform = __all(!active || tile == t0);
+    if (uniform) {
+        float m = active ? v : 0.0f;
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        if (__lane_id() == 0 && __any(active)) atomic_max_pos(amax + t0, m);
+    } else if (active) {
+        atomic_max_pos(amax + tile, v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal-block LU (getrf_nopiv, getrf_nopiv.hpp:124-141 with NSMALL panels)
+//
+// One workgroup of 512 threads factors an n x n block (n <= NMAX) in place.
+// Right-looking over column panels of width W held in LDS (column-major,
+// rows contiguous), the U strip L11^-1 A12 in LDS, and a register-blocked
+// 4x4 trailing update A22 -= L21 U12 read from / written to global (L2).
+// ---------------------------------------------------------------------------
+constexpr int kGetrfThreads = 512;
+
+template <int NMAX, int W>
+__global__ __launch_bounds__(kGetrfThreads) void getrf_kernel(int n, float *__restrict__ A,
+                                                              int64_t lda, int *__restrict__ info)
+{
+    __shared__ float Lc[W * NMAX];   // panel, Lc[c * NMAX + r]
+    __shared__ float Us[W * NMAX];   // U strip, Us[r * NMAX + c]
+    const int tid = threadIdx.x;
+    constexpr int NT = kGetrfThreads;
+    int bad = 0;
+    for (int k = 0; k < n; k += W) {
+        const int w = n - k < W ? n - k : W;
+        const int h = n - k;
+        float *Ak = A + static_cast<int64_t>(k) * lda + k;
+        for (int idx = tid; idx < h * w; idx += NT) {
+            int r = idx % h, c = idx / h;
+            Lc[c * NMAX + r] = Ak[static_cast<int64_t>(c) * lda + r];
+        }
+        __syncthreads();
+        // unblocked right-looking LU of the h x w panel (getrf_nopiv_small,
+        // getrf_nopiv.hpp:30-46, continued down the rows = trsmR)
+        for (int kk = 0; kk < w; ++kk) {
+            const float piv = Lc[kk * NMAX + kk];
+            const float inv = 1.0f / piv;
+            if (tid == 0 && bad == 0 && !(piv != 0.0f && isfinite(piv))) bad = k + kk + 1;
+            for (int r = kk + 1 + tid; r < h; r += NT) {
+                float l = Lc[kk * NMAX + r] * inv;
+                Lc[kk * NMAX + r] = l;
+                for (int c = kk + 1; c < w; ++c)
+                    Lc[c * NMAX + r] = Lc[c * NMAX + r] - l * Lc[c * NMAX + kk];
+            }
+            __syncthreads();
+        }
+        for (int idx = tid; idx < h * w; idx += NT) {
+            int r = idx % h, c = idx / h;
+            Ak[static_cast<int64_t>(c) * lda + r] = Lc[c * NMAX + r];
+        }
+        const int wr = n - k - w;   // columns right of the panel
+        // U strip: x := L11^-1 x per column (strsm "L","L","N","U" order)
+        for (int c = tid; c < wr; c += NT) {
+            float x[W];
+            float *col = Ak + static_cast<int64_t>(w + c) * lda;
+#pragma unroll
+            for (int r = 0; r < W; ++r) x[r] = r < w ? col[r] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < W; ++r)
+#pragma unroll
+                for (int i = r + 1; i < W; ++i) x[i] = x[i] - x[r] * Lc[r * NMAX + i];
+#pragma unroll
+            for (int r = 0; r < W; ++r)
+                if (r < w) {
+                    col[r] = x[r];
+                    Us[r * NMAX + c] = x[r];
+                }
+        }
+        __syncthreads();
+        // trailing update, 4x4 register tiles; rows map fastest for coalescing
+        const int hr = h - w;
+        if (hr > 0 && wr > 0) {
+            const int rt = (hr + 3) / 4, ct = (wr + 3) / 4;
+            for (int t = tid; t < rt * ct; t += NT) {
+                const int r0 = w + 4 * (t % rt), c0 = 4 * (t / rt);
+                float acc[4][4] = {};
+                for (int kk = 0; kk < w; ++kk) {
+                    float l[4], u[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        l[q] = Lc[kk * NMAX + (r0 + q < h ? r0 + q : h - 1)];
+                        u[q] = Us[kk * NMAX + (c0 + q < wr ? c0 + q : wr - 1)];
+                    }
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) acc[b][a] = fmaf(l[a], u[b], acc[b][a]);
+                }
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    if (c0 + b >= wr) break;
+                    float *col = Ak + static_cast<int64_t>(w + c0 + b) * lda;
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+                        if (r0 + a < h) col[r0 + a] = col[r0 + a] - acc[b][a];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && info != nullptr) *info = bad;
+}
+
+// ---------------------------------------------------------------------------
+// L panel TRSM: L(m x b) := L U^-1, U upper non-unit (update_left_panels,
+// panel_trf.hpp:143-157 -> strsm "R","U","N","N").  One thread per row; the
+// row is solved in column blocks of CB held in registers; each block's
+// coefficients U(0:jb+CB, jb:jb+CB) are staged in LDS.  Per element the
+// subtraction order is l = 0..j-1 followed by the reciprocal multiply, the
+// netlib order.
+// ---------------------------------------------------------------------------
+constexpr int kTrsmCB = 32;
+constexpr int kTrsmMaxB = 1024;
+
+__global__ __launch_bounds__(256) void trsm_l_kernel(int64_t m, int b, const float *__restrict__ U,
+                                                     int64_t ldu, float *__restrict__ L,
+                                                     int64_t ldl, float *__restrict__ amax, int tb)
+{
+    __shared__ float Ut[kTrsmMaxB / 2 * kTrsmCB];   // Ut[l * CB + c], l < 512 per pass
+    __shared__ float Dinv[kTrsmCB];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const bool active = i < m;
+    float *row = L + (active ? i : 0);
+    float vmax = 0.0f;
+    for (int jb = 0; jb < b; jb += kTrsmCB) {
+        const int w = b - jb < kTrsmCB ? b - jb : kTrsmCB;
+        float x[kTrsmCB];
+#pragma unroll
+        for (int c = 0; c < kTrsmCB; ++c)
+            x[c] = (active && c < w) ? row[static_cast<int64_t>(jb + c) * ldl] : 0.0f;
+        // contributions of finished columns, staged 512 rows of U at a time
+        for (int l0 = 0; l0 < jb + w; l0 += kTrsmMaxB / 2) {
+            const int l1 = (jb + w) - l0 < kTrsmMaxB / 2 ? (jb + w) : l0 + kTrsmMaxB / 2;
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < (l1 - l0) * kTrsmCB; idx += 256) {
+                int l = idx / kTrsmCB, c = idx % kTrsmCB;
+                Ut[idx] = c < w ? U[static_cast<int64_t>(jb + c) * ldu + l0 + l] : 0.0f;
+            }
+            if (l0 == 0 && threadIdx.x < kTrsmCB)
+                Dinv[threadIdx.x] = threadIdx.x < w
+                    ? 1.0f / U[static_cast<int64_t>(jb + threadIdx.x) * ldu + jb + threadIdx.x]
+                    : 0.0f;
+            __syncthreads();
+            const int lend = l1 < jb ? l1 : jb;
+            for (int l = l0; l < lend; ++l) {
+                const float xl = active ? row[static_cast<int64_t>(l) * ldl] : 0.0f;
+                const float *ur = Ut + (l - l0) * kTrsmCB;
+#pragma unroll
+                for (int c = 0; c < kTrsmCB; ++c) x[c] = x[c] - xl * ur[c];
+            }
+            if (l1 == jb + w) {
+                // diagonal block: rows jb..jb+w of U are at Ut[(jb - l0 + r) * CB + c]
+                const float *ud = Ut + (jb - l0) * kTrsmCB;
+#pragma unroll
+                for (int c = 0; c < kTrsmCB; ++c) {
+#pragma unroll
+                    for (int l = 0; l < c; ++l) x[c] = x[c] - x[l] * ud[l * kTrsmCB + c];
+                    x[c] = x[c] * Dinv[c];
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kTrsmCB; ++c)
+            if (active && c < w) {
+                row[static_cast<int64_t>(jb + c) * ldl] = x[c];
+                vmax = fmaxf(vmax, fabsf(x[c]));
+            }
+    }
+    tile_max_reduce(amax, active ? i / tb : 0, vmax, active);
+}
+
+// ---------------------------------------------------------------------------
+// U panel TRSM: U(b x n) := Lkk^-1 U, Lkk unit lower (update_right_panels,
+// panel_trf.hpp:158-171 -> strsm "L","L","N","U").  One thread per column,
+// rows solved in blocks of CB in registers; Lkk coefficients are wave-uniform
+// loads.  Element i subtracts rows k = 0..i-1 in order (netlib order).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void trsm_u_kernel(int b, int64_t n, const float *__restrict__ Lk,
+                                                     int64_t ldl, float *__restrict__ U,
+                                                     int64_t ldu, float *__restrict__ amax, int tb)
+{
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const bool active = j < n;
+    float *col = U + (active ? j : 0) * ldu;
+    float vmax = 0.0f;
+    for (int ib = 0; ib < b; ib += kTrsmCB) {
+        const int w = b - ib < kTrsmCB ? b - ib : kTrsmCB;
+        float x[kTrsmCB];
+#pragma unroll
+        for (int r = 0; r < kTrsmCB; ++r) x[r] = (active && r < w) ? col[ib + r] : 0.0f;
+        for (int l = 0; l < ib; ++l) {
+            const float xl = active ? col[l] : 0.0f;
+            const float *lc = Lk + static_cast<int64_t>(l) * ldl + ib;
+#pragma unroll
+            for (int r = 0; r < kTrsmCB; ++r)
+                if (r < w) x[r] = x[r] - xl * lc[r];
+        }
+#pragma unroll
+        for (int r = 0; r < kTrsmCB; ++r) {
+            const float *lc = Lk + static_cast<int64_t>(ib + r) * ldl + ib;
+#pragma unroll
+            for (int q = r + 1; q < kTrsmCB; ++q)
+                if (q < w) x[q] = x[q] - x[r] * lc[q];
+        }
+#pragma unroll
+        for (int r = 0; r < kTrsmCB; ++r)
+            if (active && r < w) {
+                col[ib + r] = x[r];
+                vmax = fmaxf(vmax, fabsf(x[r]));
+            }
+    }
+    tile_max_reduce(amax, active ? j / tb : 0, vmax, active);
+}
+
+// ---------------------------------------------------------------------------
+// per-tile max|x| of a panel (side 0: m x k L panel, tiles by row / tb;
+// side 1: k x m U panel, tiles by column / tb)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void tile_amax_kernel(int side, int64_t m, int64_t k, int tb,
+                                                        const float *__restrict__ src, int64_t lds,
+                                                        float *__restrict__ amax)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const bool active = t < m;
+    float v = 0.0f;
+    if (active) {
+        if (side == 0)
+            for (int64_t c = 0; c < k; ++c) v = fmaxf(v, fabsf(src[c * lds + t]));
+        else
+            for (int64_t r = 0; r < k; ++r) v = fmaxf(v, fabsf(src[t * lds + r]));
+    }
+    tile_max_reduce(amax, active ? t / tb : 0, v, active);
+}
+
+__global__ void fill_f32_kernel(float *p, int64_t cnt, float v)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < cnt) p[t] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Quantizer (convert_left_panels / convert_right_panels, panel_trf.hpp:173-233)
+// q = round(x * 2^e) with e from the tile's max (0 for plain FP16).
+// side 0: L panel m x k col-major -> dst[i*ldd + kk] (transposed through LDS)
+// side 1: U panel k x m col-major -> dst[j*ldd + kk]
+// ---------------------------------------------------------------------------
+template <int FMT>
+__device__ __forceinline__ int fmt_exponent(const float *amax, int64_t tile)
+{
+    if (FMT == HPLAI_FMT_F16) return 0;
+    const float a = amax[tile];
+    return FMT == HPLAI_FMT_E4M3 ? tile_exponent(a, kE4m3MrFrac, kE4m3Er)
+                                 : tile_exponent(a, kF16MrFrac, kF16Er);
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void quant_l_kernel(int64_t m, int64_t k, int tb,
+                                                      const float *__restrict__ src, int64_t lds,
+                                                      void *__restrict__ dst, int64_t ldd,
+                                                      const float *__restrict__ amax,
+                                                      int32_t *__restrict__ exps)
+{
+    __shared__ float T[64][65];
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 64;
+    const int64_t c0 = static_cast<int64_t>(blockIdx.y) * 64;
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    for (int c = ty; c < 64; c += 4) {
+        const int64_t gr = r0 + tx, gc = c0 + c;
+        T[c][tx] = (gr < m && gc < k) ? src[gc * lds + gr] : 0.0f;
+    }
+    __syncthreads();
+    // each thread writes 4 consecutive k values of one row (16 rows x 16 chunks per pass)
+    for (int p = threadIdx.x; p < 64 * 16; p += 256) {
+        const int r = p / 16, kc = (p % 16) * 4;
+        const int64_t gr = r0 + r;
+        if (gr >= m || c0 + kc >= k) continue;
+        const int e = fmt_exponent<FMT>(amax, gr / tb);
+        if (exps != nullptr && (gr % tb) == 0 && c0 == 0 && kc == 0) exps[gr / tb] = e;
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = ldexp_fast(T[kc + q][r], e);
+        const int nk = k - (c0 + kc) < 4 ? static_cast<int>(k - (c0 + kc)) : 4;
+        if (FMT == HPLAI_FMT_E4M3) {
+            uint8_t *d8 = static_cast<uint8_t *>(dst) + gr * ldd + c0 + kc;
+            uint32_t w = f32x4_to_e4m3x4(v[0], v[1], v[2], v[3]);
+            for (int q = 0; q < nk; ++q) d8[q] = static_cast<uint8_t>(w >> (8 * q));
+        } else {
+            uint16_t *d16 = static_cast<uint16_t *>(dst) + gr * ldd + c0 + kc;
+            for (int q = 0; q < nk; ++q) d16[q] = f32_to_f16_bits(v[q]);
+        }
+    }
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void quant_u_kernel(int64_t m, int64_t k, int tb,
+                                                      const float *__restrict__ src, int64_t lds,
+                                                      void *__restrict__ dst, int64_t ldd,
+                                                      const float *__restrict__ amax,
+                                                      int32_t *__restrict__ exps)
+{
+    // thread handles 4 consecutive rows (k index) of one column j
+    const int64_t kq = ceil_div(k, 4);
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t j = t / kq, r = (t % kq) * 4;
+    if (j >= m) return;
+    const int e = fmt_exponent<FMT>(amax, j / tb);
+    if (exps != nullptr && r == 0 && (j % tb) == 0) exps[j / tb] = e;
+    const int nk = k - r < 4 ? static_cast<int>(k - r) : 4;
+    float v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = q < nk ? ldexp_fast(src[j * lds + r + q], e) : 0.0f;
+    if (FMT == HPLAI_FMT_E4M3) {
+        uint8_t *d8 = static_cast<uint8_t *>(dst) + j * ldd + r;
+        uint32_t w = f32x4_to_e4m3x4(v[0], v[1], v[2], v[3]);
+        for (int q = 0; q < nk; ++q) d8[q] = static_cast<uint8_t>(w >> (8 * q));
+    } else {
+        uint16_t *d16 = static_cast<uint16_t *>(dst) + j * ldd + r;
+        for (int q = 0; q < nk; ++q) d16[q] = f32_to_f16_bits(v[q]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+int launch_getrf(int n, float *A, int64_t lda, int *info, hipStream_t s)
+{
+    if (n <= 0) return HPLAI_OK;
+    if (n <= 512)
+        hipLaunchKernelGGL((getrf_kernel<512, 32>), dim3(1), dim3(kGetrfThreads), 0, s, n, A, lda, info);
+    else
+        hipLaunchKernelGGL((getrf_kernel<1024, 16>), dim3(1), dim3(kGetrfThreads), 0, s, n, A, lda, info);
+    HPLAI_HIP(hipGetLastError());
+    return HPLAI_OK;
+}
+
+int launch_trsm_l(int64_t m, int b, const float *U, int64_t ldu, float *L, int64_t ldl, float *amax,
+                  int tb, hipStream_t s)
+{
+    if (m <= 0 || b <= 0) return HPLAI_OK;
+    hipLaunchKernelGGL(trsm_l_kernel, dim3(static_cast<unsigned>(ceil_div(m, 256))), dim3(256), 0, s,
+                       m, b, U, ldu, L, ldl, amax, tb);
+    HPLAI_HIP(hipGetLastError());
+    return HPLAI_OK;
+}
+
+int launch_trsm_u(int b, int64_t n, const float *Lkk, int64_t ldl, float *U, int64_t ldu,
+                  float *amax, int tb, hipStream_t s)
+{
+    if (n <= 0 || b <= 0) return HPLAI_OK;
+    hipLaunchKernelGGL(trsm_u_kernel, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s,
+                       b, n, Lkk, ldl, U, ldu, amax, tb);
+    HPLAI_HIP(hipGetLastError());
+    return HPLAI_OK;
+}
+
+int launch_tile_amax(int side, int64_t m, int64_t k, int tb, const float *src, int64_t lds,
+                     float *amax, hipStream_t s)
+{
+    if (m <= 0 || k <= 0) return HPLAI_OK;
+    hipLaunchKernelGGL(tile_amax_kernel, dim3(static_cast<unsigned>(ceil_div(m, 256))), dim3(256), 0,
+                       s, side, m, k, tb, src, lds, amax);
+    HPLAI_HIP(hipGetLastError());
+    return HPLAI_OK;
+}
+
+int launch_fill_f32(float *p, int64_t cnt, float v, hipStream_t s)
+{
+    if (cnt <= 0) return HPLAI_OK;
+    hipLaunchKernelGGL(fill_f32_kernel, dim3(static_cast<unsigned>(ceil_div(cnt, 256))), dim3(256), 0,
+                       s, p, cnt, v);
+    HPLAI_HIP(hipGetLastError());
+    return HPLAI_OK;
+}
+
+template <int FMT>
+static int quant_dispatch(int side, int64_t m, int64_t k, int tb, const float *src, int64_t lds,
+                          void *dst, int64_t ldd, const float *amax, int32_t *exps, hipStream_t s)
+{
+    if (side == 0) {
+        dim3 grid(static_cast<unsigned>(ceil_div(m, 64)), static_cast<unsigned>(ceil_div(k, 64)));
+        hipLaunchKernelGGL(quant_l_kernel<FMT>, grid, dim3(256), 0, s, m, k, tb, src, lds, dst, ldd,
+                           amax, exps);
+    } else {
+        const int64_t thr = m * ceil_div(k, 4);
+        hipLaunchKernelGGL(quant_u_kernel<FMT>, dim3(static_cast<unsigned>(ceil_div(thr, 256))),
+                           dim3(256), 0, s, m, k, tb, src, lds, dst, ldd, amax, exps);
+    }
+    HPLAI_HIP(hipGetLastError());
+    return HPLAI_OK;
+}
+
+int launch_quantize(int fmt, int side, int64_t m, int64_t k, int tb, const float *src, int64_t lds,
+                    void *dst, int64_t ldd, const float *amax, int32_t *exps, hipStream_t s)
+{
+    if (m <= 0 || k <= 0) return HPLAI_OK;
+    switch (fmt) {
+    case HPLAI_FMT_F16:
+        return quant_dispatch<HPLAI_FMT_F16>(side, m, k, tb, src, lds, dst, ldd, amax, exps, s);
+    case HPLAI_FMT_F16_TILE:
+        return quant_dispatch<HPLAI_FMT_F16_TILE>(side, m, k, tb, src, lds, dst, ldd, amax, exps, s);
+    case HPLAI_FMT_E4M3:
+        return quant_dispatch<HPLAI_FMT_E4M3>(side, m, k, tb, src, lds, dst, ldd, amax, exps, s);
+    default:
+        set_error("quantize: unknown format %d", fmt);
+        return HPLAI_EINVAL;
+    }
+}
+
+}  // namespace hplai

@@ -1,0 +1,442 @@
+// solve.hip — host orchestration of the whole HPL-AI solve on one GPU.
+//
+// Replaces the driver's timed region (main.cpp:379-433): RDMAPanelLU::run
+// (panel_trf.hpp:703-856) followed by iterative_refinement
+// (iterative_refinement.hpp:18-57).  The whole matrix lives in HBM as FP32
+// (column-major, lda = n rounded to 64), so the 2-D block-cyclic grid and its
+// broadcasts become in-device hand-offs through the quantized panel buffers.
+//
+// LU schedule, two levels:
+//  - inner block b (the reference's b): diagonal LU, FP32 TRSMs, quantization;
+//  - outer block B = outer * b: the trailing matrix is updated once per outer
+//    step with K = B, which divides its HBM traffic by `outer`.  With outer = 1
+//    this is exactly the reference's right-looking schedule.
+// Per outer step (columns c0 .. c0+B):
+//   1. panel: for each inner block t: getrf, L := A U^-1 (rows below),
+//      quantize L; U := L^-1 A inside the panel, quantize, and update the rest
+//      of the panel (K = b);
+//   2. block row: for each t: U12_t := L_tt^-1 A12_t, quantize, update the
+//      block rows below it inside the block row (K = b);
+//   3. trailing: A22 -= Lq(:, 0..B) Uq(0..B, :) (K = B).
+// With lookahead, step 3 first updates the next outer panel's columns, then
+// the panel factorization of step k+1 runs on a second stream while the rest
+// of step 3 runs on the main stream.
+#include <chrono>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace hplai;
+
+namespace {
+
+struct Buffers {
+    float *A = nullptr;
+    double *d = nullptr, *rhs = nullptr, *x = nullptr, *r = nullptr, *work = nullptr, *norms = nullptr;
+    void *Lq[2] = {nullptr, nullptr}, *Uq[2] = {nullptr, nullptr};
+    int32_t *eL[2] = {nullptr, nullptr}, *eU[2] = {nullptr, nullptr};
+    float *amaxL[2] = {nullptr, nullptr}, *amaxU[2] = {nullptr, nullptr};
+    int *info = nullptr;
+    double *chk = nullptr;
+};
+
+void free_buffers(Buffers &B, bool keep_factors)
+{
+    auto f = [](void *p) {
+        if (p) (void)hipFree(p);
+    };
+    if (!keep_factors) {
+        f(B.A);
+        f(B.x);
+    }
+    f(B.d);
+    f(B.rhs);
+    f(B.r);
+    f(B.work);
+    f(B.norms);
+    f(B.info);
+    f(B.chk);
+    for (int i = 0; i < 2; ++i) {
+        f(B.Lq[i]);
+        f(B.Uq[i]);
+        f(B.eL[i]);
+        f(B.eU[i]);
+        f(B.amaxL[i]);
+        f(B.amaxU[i]);
+    }
+}
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct LU {
+    int64_t n, lda, ldq;
+    int b, outer, fmt, kpad, es;
+    bool lazy;
+    uint64_t seed;
+    int64_t ntiles;
+    Buffers *B;
+    hipStream_t main, side;
+    hipEvent_t *ev;
+    int num_cu;
+
+    bool tiled() const { return fmt != HPLAI_FMT_F16; }
+    float *a(int64_t i, int64_t j) const { return B->A + j * lda + i; }
+    uint8_t *lq(int buf, int64_t row, int t) const
+    {
+        return static_cast<uint8_t *>(B->Lq[buf]) + (row * ldq + static_cast<int64_t>(t) * kpad) * es;
+    }
+    uint8_t *uq(int buf, int64_t col, int t) const
+    {
+        return static_cast<uint8_t *>(B->Uq[buf]) + (col * ldq + static_cast<int64_t>(t) * kpad) * es;
+    }
+    int32_t *el(int buf, int t, int64_t row) const
+    {
+        return tiled() ? B->eL[buf] + t * ntiles + row / b : nullptr;
+    }
+    int32_t *eu(int buf, int t, int64_t col) const
+    {
+        return tiled() ? B->eU[buf] + t * ntiles + col / b : nullptr;
+    }
+    float *aml(int buf, int t, int64_t row) const
+    {
+        return tiled() ? B->amaxL[buf] + t * ntiles + row / b : nullptr;
+    }
+    float *amu(int buf, int t, int64_t col) const
+    {
+        return tiled() ? B->amaxU[buf] + t * ntiles + col / b : nullptr;
+    }
+
+    int gemm(int buf, int t0, int nsl, int64_t row0, int64_t col0, int64_t m, int64_t nn,
+             hipStream_t s) const
+    {
+        if (m <= 0 || nn <= 0) return HPLAI_OK;
+        return launch_schur(fmt, m, nn, nsl, kpad, lq(buf, row0, t0), ldq, el(buf, t0, row0), ntiles,
+                            uq(buf, col0, t0), ldq, eu(buf, t0, col0), ntiles, b, a(row0, col0), lda,
+                            num_cu, s);
+    }
+
+    // Lazy initialisation (-d): add the generated a_ij to the partial sums of
+    // an m x k block when it joins a panel (LazyInitializer, lazy_init.hpp:92-227).
+    int lazy_init(int64_t i0, int64_t j0, int64_t m, int64_t k, hipStream_t s) const
+    {
+        if (!lazy || m <= 0 || k <= 0) return HPLAI_OK;
+        return launch_gen_matrix(n, seed, i0, j0, m, k, B->d, a(i0, j0), lda, 1, s);
+    }
+
+    // Step 1 for the outer block at c0 with sw inner blocks.
+    int panel(int64_t c0, int sw, int buf, hipStream_t s) const
+    {
+        int rc = HPLAI_OK;
+        const int64_t Bw = static_cast<int64_t>(sw) * b;
+        if (tiled()) {
+            rc = launch_fill_f32(B->amaxL[buf], static_cast<int64_t>(outer) * ntiles, 0.0f, s);
+            if (rc == HPLAI_OK) rc = launch_fill_f32(B->amaxU[buf], static_cast<int64_t>(outer) * ntiles, 0.0f, s);
+            if (rc) return rc;
+        }
+        for (int t = 0; t < sw && rc == HPLAI_OK; ++t) {
+            const int64_t c = c0 + static_cast<int64_t>(t) * b;
+            rc = lazy_init(c, c, n - c, b, s);   // block column: diagonal block + L panel
+            if (rc == HPLAI_OK) rc = launch_getrf(b, a(c, c), lda, B->info + c / b, s);
+            const int64_t mrow = n - c - b;
+            if (rc || mrow <= 0) continue;
+            rc = launch_trsm_l(mrow, b, a(c, c), lda, a(c + b, c), lda, aml(buf, t, c + b), b, s);
+            if (rc == HPLAI_OK)
+                rc = launch_quantize(fmt, 0, mrow, b, b, a(c + b, c), lda, lq(buf, c + b, t), ldq,
+                                     aml(buf, t, c + b), el(buf, t, c + b), s);
+            const int64_t ncols = c0 + Bw - (c + b);
+            if (rc || ncols <= 0) continue;
+            rc = lazy_init(c, c + b, b, ncols, s);   // block row inside the panel
+            if (rc == HPLAI_OK)
+                rc = launch_trsm_u(b, ncols, a(c, c), lda, a(c, c + b), lda, amu(buf, t, c + b), b, s);
+            if (rc == HPLAI_OK)
+                rc = launch_quantize(fmt, 1, ncols, b, b, a(c, c + b), lda, uq(buf, c + b, t), ldq,
+                                     amu(buf, t, c + b), eu(buf, t, c + b), s);
+            if (rc == HPLAI_OK) rc = gemm(buf, t, 1, c + b, c + b, mrow, ncols, s);
+        }
+        return rc;
+    }
+
+    // Step 2: block row U12 of the outer block at c0.
+    int block_row(int64_t c0, int sw, int buf, hipStream_t s) const
+    {
+        int rc = HPLAI_OK;
+        const int64_t Bw = static_cast<int64_t>(sw) * b;
+        const int64_t col0 = c0 + Bw, ncols = n - col0;
+        if (ncols <= 0) return rc;
+        for (int t = 0; t < sw && rc == HPLAI_OK; ++t) {
+            const int64_t c = c0 + static_cast<int64_t>(t) * b;
+            rc = lazy_init(c, col0, b, ncols, s);   // block row right of the outer panel
+            if (rc == HPLAI_OK)
+                rc = launch_trsm_u(b, ncols, a(c, c), lda, a(c, col0), lda, amu(buf, t, col0), b, s);
+            if (rc == HPLAI_OK)
+                rc = launch_quantize(fmt, 1, ncols, b, b, a(c, col0), lda, uq(buf, col0, t), ldq,
+                                     amu(buf, t, col0), eu(buf, t, col0), s);
+            if (rc == HPLAI_OK && t + 1 < sw) rc = gemm(buf, t, 1, c + b, col0, col0 - (c + b), ncols, s);
+        }
+        return rc;
+    }
+
+    int run(bool lookahead) const
+    {
+        const int64_t Bfull = static_cast<int64_t>(outer) * b;
+        int rc = HPLAI_OK;
+        int64_t c0 = 0;
+        int step = 0;
+        int sw = static_cast<int>(std::min<int64_t>(outer, (n - c0) / b));
+        rc = panel(c0, sw, 0, main);
+        while (rc == HPLAI_OK && c0 < n) {
+            const int buf = step & 1;
+            const int64_t Bw = static_cast<int64_t>(sw) * b;
+            rc = block_row(c0, sw, buf, main);
+            if (rc) break;
+            const int64_t r0 = c0 + Bw, rest = n - r0;
+            const int64_t c1 = r0;
+            const int sw1 = rest > 0 ? static_cast<int>(std::min<int64_t>(outer, rest / b)) : 0;
+            if (rest > 0) {
+                if (lookahead) {
+                    // next panel's columns first, then hand them to the side stream
+                    const int64_t B1 = static_cast<int64_t>(sw1) * b;
+                    rc = gemm(buf, 0, sw, r0, r0, rest, B1, main);
+                    if (rc) break;
+                    HPLAI_HIP(hipEventRecord(ev[0], main));
+                    HPLAI_HIP(hipStreamWaitEvent(side, ev[0], 0));
+                    rc = panel(c1, sw1, buf ^ 1, side);
+                    if (rc) break;
+                    HPLAI_HIP(hipEventRecord(ev[1], side));
+                    rc = gemm(buf, 0, sw, r0, r0 + B1, rest, rest - B1, main);
+                    if (rc) break;
+                    HPLAI_HIP(hipStreamWaitEvent(main, ev[1], 0));
+                } else {
+                    rc = gemm(buf, 0, sw, r0, r0, rest, rest, main);
+                    if (rc == HPLAI_OK) rc = panel(c1, sw1, buf ^ 1, main);
+                    if (rc) break;
+                }
+            }
+            c0 = c1;
+            sw = sw1;
+            ++step;
+            (void)Bfull;
+        }
+        return rc;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int hplai_release(hplai_ctx ctx)
+{
+    if (ctx == nullptr) return HPLAI_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->A) (void)hipFree(ctx->A);
+    if (ctx->x) (void)hipFree(ctx->x);
+    ctx->A = nullptr;
+    ctx->x = nullptr;
+    ctx->n = ctx->lda = 0;
+    return HPLAI_OK;
+}
+
+int hplai_solution_x(hplai_ctx ctx, double *x_host)
+{
+    HPLAI_CHECK(ctx != nullptr && ctx->x != nullptr && x_host != nullptr,
+                "solution_x: no kept solution (run hplai_solve with keep_factors)");
+    HPLAI_HIP(hipSetDevice(ctx->device));
+    HPLAI_HIP(hipMemcpy(x_host, ctx->x, sizeof(double) * ctx->n, hipMemcpyDeviceToHost));
+    return HPLAI_OK;
+}
+
+int hplai_solution_factors(hplai_ctx ctx, float *lu_host, int64_t ldh)
+{
+    HPLAI_CHECK(ctx != nullptr && ctx->A != nullptr && lu_host != nullptr && ldh >= ctx->n,
+                "solution_factors: no kept factors or ldh < n");
+    HPLAI_HIP(hipSetDevice(ctx->device));
+    HPLAI_HIP(hipMemcpy2D(lu_host, sizeof(float) * ldh, ctx->A, sizeof(float) * ctx->lda,
+                          sizeof(float) * ctx->n, ctx->n, hipMemcpyDeviceToHost));
+    return HPLAI_OK;
+}
+
+int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
+{
+    HPLAI_CHECK(ctx != nullptr && p != nullptr && res != nullptr, "hplai_solve: NULL argument");
+    HPLAI_HIP(hipSetDevice(ctx->device));
+    memset(res, 0, sizeof(*res));
+    const int64_t n = p->n;
+    const int b = p->b;
+    const int fmt = p->fmt;
+    const int outer = p->outer > 0 ? p->outer : 1;
+    const int maxit = p->maxit > 0 ? (p->maxit < 64 ? p->maxit : 64) : 50;
+    HPLAI_CHECK(n > 0 && b > 0 && n % b == 0, "hplai_solve: need n %% b == 0 (n=%lld b=%d)",
+                (long long)n, b);
+    HPLAI_CHECK(b <= 1024, "hplai_solve: b=%d > 1024", b);
+    HPLAI_CHECK(fmt == HPLAI_FMT_F16 || fmt == HPLAI_FMT_F16_TILE || fmt == HPLAI_FMT_E4M3,
+                "hplai_solve: unknown fmt %d", fmt);
+    const int es = fmt == HPLAI_FMT_E4M3 ? 1 : 2;
+    const int bk = 128 / es;
+    if (fmt != HPLAI_FMT_F16)
+        HPLAI_CHECK(b % 128 == 0, "hplai_solve: tile-scaled formats need b %% 128 == 0 (b=%d)", b);
+    hplai_release(ctx);
+
+    LU lu;
+    Buffers B;
+    lu.n = n;
+    lu.lda = (n + 63) / 64 * 64;
+    lu.b = b;
+    lu.outer = outer;
+    lu.fmt = fmt;
+    lu.es = es;
+    lu.kpad = (b + bk - 1) / bk * bk;
+    lu.ldq = static_cast<int64_t>(outer) * lu.kpad;
+    lu.ntiles = n / b;
+    lu.lazy = p->lazy != 0;
+    lu.seed = p->seed;
+    lu.B = &B;
+    lu.main = nullptr;
+    lu.side = ctx->side;
+    lu.ev = ctx->ev;
+    lu.num_cu = ctx->num_cu;
+    hipStream_t s = nullptr;
+
+    int rc = HPLAI_OK;
+#define ALLOC(ptr, bytes)                                                             \
+    do {                                                                              \
+        hipError_t _e = hipMalloc(reinterpret_cast<void **>(&(ptr)), (bytes));        \
+        if (_e != hipSuccess) {                                                       \
+            rc = hip_fail(_e, "hplai_solve: hipMalloc " #ptr);                        \
+            free_buffers(B, false);                                                   \
+            return rc;                                                                \
+        }                                                                             \
+    } while (0)
+    ALLOC(B.A, sizeof(float) * lu.lda * n);
+    ALLOC(B.d, sizeof(double) * n);
+    ALLOC(B.rhs, sizeof(double) * n);
+    ALLOC(B.x, sizeof(double) * n);
+    ALLOC(B.r, sizeof(double) * n);
+    ALLOC(B.work, sizeof(double) * otf_gemv_work_doubles(n));
+    ALLOC(B.norms, sizeof(double) * 4);
+    ALLOC(B.info, sizeof(int) * lu.ntiles);
+    ALLOC(B.chk, sizeof(double) * 3);
+    for (int i = 0; i < 2; ++i) {
+        const size_t qbytes = static_cast<size_t>(n) * lu.ldq * es;
+        ALLOC(B.Lq[i], qbytes);
+        ALLOC(B.Uq[i], qbytes);
+        HPLAI_HIP(hipMemsetAsync(B.Lq[i], 0, qbytes, s));
+        HPLAI_HIP(hipMemsetAsync(B.Uq[i], 0, qbytes, s));
+        if (fmt != HPLAI_FMT_F16) {
+            ALLOC(B.eL[i], sizeof(int32_t) * outer * lu.ntiles);
+            ALLOC(B.eU[i], sizeof(int32_t) * outer * lu.ntiles);
+            ALLOC(B.amaxL[i], sizeof(float) * outer * lu.ntiles);
+            ALLOC(B.amaxU[i], sizeof(float) * outer * lu.ntiles);
+        }
+    }
+#undef ALLOC
+    HPLAI_HIP(hipMemsetAsync(B.info, 0, sizeof(int) * lu.ntiles, s));
+
+    // ---- generation (main.cpp:256-275), outside the timed region ----
+    HPLAI_HIP(hipDeviceSynchronize());
+    double t0 = now_s();
+    if (rc == HPLAI_OK) rc = launch_gen_diag(n, p->seed, 0, n, B.d, s);
+    if (lu.lazy) {
+        // pmatgen0 (matgen.hpp:80-105): partial sums start at zero
+        HPLAI_HIP(hipMemsetAsync(B.A, 0, sizeof(float) * lu.lda * n, s));
+    } else if (rc == HPLAI_OK) {
+        rc = launch_gen_matrix(n, p->seed, 0, 0, n, n, B.d, B.A, lu.lda, 0, s);
+    }
+    if (rc == HPLAI_OK) rc = launch_gen_rhs(n, p->seed, 0, n, B.rhs, s);
+    HPLAI_HIP(hipDeviceSynchronize());
+    double t1 = now_s();
+    res->time_gen = t1 - t0;
+
+    // ---- LU ----
+    if (rc == HPLAI_OK) rc = lu.run(p->lookahead != 0);
+    HPLAI_HIP(hipDeviceSynchronize());
+    double t2 = now_s();
+    res->time_lu = t2 - t1;
+    if (rc == HPLAI_OK) {
+        std::vector<int> info(lu.ntiles);
+        HPLAI_HIP(hipMemcpy(info.data(), B.info, sizeof(int) * lu.ntiles, hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < lu.ntiles; ++q)
+            if (info[q] != 0) {
+                set_error("hplai_solve: zero or non-finite pivot at column %lld",
+                          (long long)(q * b + info[q] - 1));
+                rc = HPLAI_EPIVOT;
+                break;
+            }
+    }
+
+    // ---- iterative refinement (iterative_refinement.hpp:18-57) ----
+    int steps = 0;
+    bool converged = false;
+    if (rc == HPLAI_OK) {
+        double hn[4];
+        rc = launch_norm_inf(n, B.rhs, B.norms + 0, s);
+        if (rc == HPLAI_OK) rc = launch_norm_inf(n, B.d, B.norms + 1, s);
+        if (rc == HPLAI_OK) rc = launch_vec_div(n, B.rhs, B.d, B.x, s);   // x0 = b ./ diag(A)
+        HPLAI_HIP(hipMemcpy(hn, B.norms, sizeof(double) * 2, hipMemcpyDeviceToHost));
+        const double normb = hn[0], norma = 2.0 * hn[1];   // hpl_infnorm, panel_norm.hpp:78-84
+        for (int it = 0; it < maxit && rc == HPLAI_OK; ++it) {
+            HPLAI_HIP(hipMemcpyAsync(B.r, B.rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            rc = launch_norm_inf(n, B.x, B.norms + 2, s);
+            if (rc == HPLAI_OK) rc = launch_otf_gemv(n, p->seed, B.d, -1.0, B.x, B.r, B.work, s);
+            if (rc == HPLAI_OK) rc = launch_norm_inf(n, B.r, B.norms + 3, s);
+            if (rc) break;
+            HPLAI_HIP(hipMemcpy(hn + 2, B.norms + 2, sizeof(double) * 2, hipMemcpyDeviceToHost));
+            const double normx = hn[2], normr = hn[3];
+            const double h = normr / (norma * normx + normb) * 1. / (static_cast<double>(n) * DBL_EPSILON / 2);
+            res->trace_resid[it] = normr;
+            res->trace_harness[it] = h;
+            steps = it + 1;
+            if (p->verbose) {
+                printf("# iterative refinement: step=%3d, residual=%20.16e hpl-harness=%f\n", it, normr, h);
+                fflush(stdout);
+            }
+            if (h < 16.) {
+                res->residual = normr;
+                res->harness = h;
+                converged = true;
+                break;
+            }
+            rc = launch_trsv_lower(n, b, B.A, lu.lda, B.r, s);
+            if (rc == HPLAI_OK) rc = launch_trsv_upper(n, b, B.A, lu.lda, B.r, s);
+            if (rc == HPLAI_OK) rc = launch_vec_add(n, B.r, B.x, s);
+        }
+    }
+    HPLAI_HIP(hipDeviceSynchronize());
+    double t3 = now_s();
+    res->time_ir = t3 - t2;
+    res->time_total = t3 - t1;
+    res->ir_steps = steps;
+    const double dn = static_cast<double>(n);
+    res->gflops = (2. / 3 * dn * dn * dn + 3. / 2 * dn * dn) / res->time_total * 1e-9;
+    if (!converged) {
+        res->residual = -1.0;
+        res->harness = -1.0;
+    }
+    if (rc == HPLAI_OK) {
+        rc = launch_checksum(n, B.A, lu.lda, B.chk, s);
+        if (rc == HPLAI_OK)
+            HPLAI_HIP(hipMemcpy(res->checksum, B.chk, sizeof(double) * 3, hipMemcpyDeviceToHost));
+    }
+    const bool keep = p->keep_factors != 0 && rc == HPLAI_OK;
+    if (keep) {
+        ctx->A = B.A;
+        ctx->x = B.x;
+        ctx->n = n;
+        ctx->lda = lu.lda;
+    }
+    free_buffers(B, keep);
+    if (rc == HPLAI_OK && !converged) {
+        set_error("hplai_solve: harness >= 16 after %d IR steps", steps);
+        rc = HPLAI_ENOCONV;
+    }
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// driver.cpp — `driver.out n b P [flags]`, the reference's command line
+// (main.cpp:33-131, 144-452) over the C ABI in include/hplai.h.
+//
+// One MI355X holds the whole matrix, so P (the process-grid rows) is accepted
+// and reported but no grid is built; -r (RDMA broadcast) and -not (2-D
+// block-cyclic instead of tile layout) describe communication and layout
+// choices that do not exist on one device and are accepted as no-ops.
+// Output lines follow the reference formats so runs diff directly:
+//   "# iterative refinement: step=%3d, residual=%20.16e hpl-harness=%f"
+//   "%.9f sec. %.9f GFlop/s resid = %20.15e hpl-harness = %.9f"
+//   "check %22.17e %22.17e, %22.17e"   (--checksum)
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+
+#include "hplai.h"
+
+static void usage()
+{
+    std::printf(
+        "HPL-AI for one MI355X\n\n"
+        "usage: ./driver.out <n> <b> <P> [arguments]\n\n"
+        "   n                 The matrix size.\n"
+        "   b                 The block size. Requires mod(n,b) = 0.\n"
+        "   P                 Process-grid rows of the reference; reported only.\n"
+        "   -not, -nt         Accepted (the layout is always one column-major matrix).\n"
+        "   -d, -l            Lazy initialisation: the trailing matrix starts at zero and\n"
+        "                     each block is regenerated and merged when it joins a panel.\n"
+        "   -r, --rdma        Accepted (no inter-process broadcast on one device).\n"
+        "   -w, --warmup      One IR step only.\n"
+        "   --checksum        Print the factor checksum line.\n"
+        "   --fmt <f>         Schur operands: f16 (default), f16t (per-tile 2^e), e4m3.\n"
+        "   --outer <s>       Panels per trailing update (default 1).\n"
+        "   --lookahead <0|1> Factor panel k+1 on a second stream (default 1).\n"
+        "   --seed <s>        Generator seed (default 42).\n"
+        "   --device <d>      HIP device (default 0).\n"
+        "   --help, -h        Print this message.\n");
+}
+
+static void print_ctime(const char *msg)
+{
+    std::time_t t;
+    std::time(&t);
+    std::printf("%s%s", msg, std::ctime(&t));
+}
+
+int main(int argc, char **argv)
+{
+    long long n = 120;
+    int b = 3, nrow = 2;
+    if (argc > 1) n = std::atoll(argv[1]);
+    if (argc > 2) b = std::atoi(argv[2]);
+    if (argc > 3) nrow = std::atoi(argv[3]);
+    hplai_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.n = n;
+    p.b = b;
+    p.seed = 42;   // main.cpp:174
+    p.fmt = HPLAI_FMT_F16;
+    p.outer = 1;
+    p.lookahead = 1;
+    p.maxit = 50;
+    p.verbose = 1;
+    bool checksum = false, warmup = false, lazy = false, rdma = false, tile = true;
+    int device = 0;
+    for (int i = 4; i < argc; ++i) {
+        const char *a = argv[i];
+        auto next = [&](void) -> const char * { return i + 1 < argc ? argv[++i] : ""; };
+        if (!std::strcmp(a, "-not") || !std::strcmp(a, "-nt") || !std::strcmp(a, "--notile")) tile = false;
+        else if (!std::strcmp(a, "-d") || !std::strcmp(a, "--delayed") || !std::strcmp(a, "-l") ||
+                 !std::strcmp(a, "--lazy-init")) lazy = true;
+        else if (!std::strcmp(a, "-r") || !std::strcmp(a, "--rdma")) rdma = true;
+        else if (!std::strcmp(a, "-w") || !std::strcmp(a, "--warmup")) warmup = true;
+        else if (!std::strcmp(a, "--checksum")) checksum = true;
+        else if (!std::strcmp(a, "--fmt")) {
+            const char *f = next();
+            if (!std::strcmp(f, "f16")) p.fmt = HPLAI_FMT_F16;
+            else if (!std::strcmp(f, "f16t")) p.fmt = HPLAI_FMT_F16_TILE;
+            else if (!std::strcmp(f, "e4m3") || !std::strcmp(f, "fp8")) p.fmt = HPLAI_FMT_E4M3;
+            else { std::fprintf(stderr, "unknown --fmt %s\n", f); return 2; }
+        }
+        else if (!std::strcmp(a, "--outer")) p.outer = std::atoi(next());
+        else if (!std::strcmp(a, "--lookahead")) p.lookahead = std::atoi(next());
+        else if (!std::strcmp(a, "--seed")) p.seed = std::strtoull(next(), nullptr, 10);
+        else if (!std::strcmp(a, "--device")) device = std::atoi(next());
+        else if (!std::strcmp(a, "--help") || !std::strcmp(a, "-h")) { usage(); return 0; }
+        else std::printf("Ignoring unknown option: %s\n", a);
+    }
+    if (warmup) p.maxit = 1;
+    p.lazy = lazy ? 1 : 0;
+
+    std::printf("n=%lld b=%d r=%d c=%d\n", n, b, 1, 1);
+    std::printf("%s %s %s %s nopack %s noskiplu ddmat\n", tile ? "tile" : "2dbc",
+                lazy ? "lazy" : "precomp", rdma ? "rdma" : "mpi", warmup ? "warmup" : "full",
+                checksum ? "checksum" : "nocheck");
+    std::printf("device=%d P=%d fmt=%s outer=%d lookahead=%d\n", device, nrow,
+                p.fmt == HPLAI_FMT_F16 ? "f16" : (p.fmt == HPLAI_FMT_F16_TILE ? "f16t" : "e4m3"),
+                p.outer, p.lookahead);
+    print_ctime("#BEGIN: ");
+    std::fflush(stdout);
+
+    hplai_ctx ctx = nullptr;
+    int rc = hplai_create(device, &ctx);
+    if (rc != HPLAI_OK) {
+        std::fprintf(stderr, "hplai_create: %s\n", hplai_last_error());
+        return 1;
+    }
+    hplai_result r;
+    rc = hplai_solve(ctx, &p, &r);
+    if (rc != HPLAI_OK && rc != HPLAI_ENOCONV) {
+        std::fprintf(stderr, "hplai_solve failed (%d): %s\n", rc, hplai_last_error());
+        hplai_destroy(ctx);
+        return 1;
+    }
+    if (checksum) std::printf("check %22.17e %22.17e, %22.17e\n", r.checksum[0], r.checksum[1], r.checksum[2]);
+    print_ctime("#END__: ");
+    std::printf("%.9f sec. %.9f GFlop/s resid = %20.15e hpl-harness = %.9f\n", r.time_total, r.gflops,
+                r.residual, r.harness);
+    std::printf("# generation %.6f s, LU %.6f s, IR %.6f s (%d residual evaluations)\n", r.time_gen,
+                r.time_lu, r.time_ir, r.ir_steps);
+    std::fflush(stdout);
+    hplai_destroy(ctx);
+    return rc == HPLAI_OK ? 0 : 3;
+}

@@ -1,0 +1,227 @@
+"""hplai — Python binding of the MI355X HPL-AI solver (libhplai_gfx950.so).
+
+A thin ctypes layer over the C ABI declared in include/hplai.h.  Device
+buffers are passed as raw pointers; torch tensors are accepted wherever a
+device pointer is expected (torch provides device memory and streams only).
+
+The native library must exist: nothing here falls back to a CPU path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libhplai_gfx950.so")
+DRIVER_PATH = os.path.join(os.path.dirname(_HERE), "bin", "driver.out")
+
+OK, EINVAL, EHIP, ENOMEM, ENOCONV, EPIVOT = 0, -1, -2, -3, -4, -5
+FMT_F16, FMT_F16_TILE, FMT_E4M3 = 0, 1, 2
+FMTS = {"f16": FMT_F16, "f16t": FMT_F16_TILE, "e4m3": FMT_E4M3, "fp8": FMT_E4M3}
+
+_i64, _u64, _int, _dbl = ctypes.c_int64, ctypes.c_uint64, ctypes.c_int, ctypes.c_double
+_vp = ctypes.c_void_p
+
+
+class HplaiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("hplai error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("n", _i64), ("b", _int), ("seed", _u64), ("fmt", _int), ("outer", _int),
+                ("lookahead", _int), ("lazy", _int), ("maxit", _int), ("verbose", _int),
+                ("keep_factors", _int)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("time_gen", _dbl), ("time_lu", _dbl), ("time_ir", _dbl), ("time_total", _dbl),
+                ("gflops", _dbl), ("ir_steps", _int), ("residual", _dbl), ("harness", _dbl),
+                ("trace_resid", _dbl * 64), ("trace_harness", _dbl * 64), ("checksum", _dbl * 3)]
+
+
+# (name, restype, argtypes) for every exported entry point of include/hplai.h
+_SIGS = [
+    ("hplai_create", _int, [_int, ctypes.POINTER(_vp)]),
+    ("hplai_destroy", _int, [_vp]),
+    ("hplai_last_error", ctypes.c_char_p, []),
+    ("hplai_version", _int, []),
+    ("hplai_gen_matrix_f32", _int, [_vp, _i64, _u64, _i64, _i64, _i64, _i64, _vp, _vp, _i64, _vp]),
+    ("hplai_gen_diag_f64", _int, [_vp, _i64, _u64, _i64, _i64, _vp, _vp]),
+    ("hplai_gen_rhs_f64", _int, [_vp, _i64, _u64, _i64, _i64, _vp, _vp]),
+    ("hplai_getrf_nopiv_f32", _int, [_vp, _int, _vp, _i64, _vp, _vp]),
+    ("hplai_trsm_lpanel_f32", _int, [_vp, _i64, _int, _vp, _i64, _vp, _i64, _vp]),
+    ("hplai_trsm_upanel_f32", _int, [_vp, _int, _i64, _vp, _i64, _vp, _i64, _vp]),
+    ("hplai_quantize", _int, [_vp, _int, _int, _i64, _i64, _int, _vp, _i64, _vp, _i64, _vp, _vp]),
+    ("hplai_schur_update", _int, [_vp, _int, _i64, _i64, _int, _int, _vp, _i64, _vp, _i64, _vp, _i64,
+                                  _vp, _i64, _int, _vp, _i64, _vp]),
+    ("hplai_otf_gemv_f64", _int, [_vp, _i64, _u64, _vp, _dbl, _vp, _vp, _vp]),
+    ("hplai_trsv_lower_f32", _int, [_vp, _i64, _int, _vp, _i64, _vp, _vp]),
+    ("hplai_trsv_upper_f32", _int, [_vp, _i64, _int, _vp, _i64, _vp, _vp]),
+    ("hplai_norm_inf_f64", _int, [_vp, _i64, _vp, _vp, _vp]),
+    ("hplai_solve", _int, [_vp, ctypes.POINTER(Params), ctypes.POINTER(Result)]),
+    ("hplai_solution_x", _int, [_vp, _vp]),
+    ("hplai_solution_factors", _int, [_vp, _vp, _i64]),
+    ("hplai_release", _int, [_vp]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load libhplai_gfx950.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libhplai_gfx950.so not built (run `make -C hpl-ai_amd` or "
+                               "__graft_entry__.build()): %s" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().hplai_last_error().decode()
+
+
+def check(rc):
+    if rc != OK:
+        raise HplaiError(rc, last_error())
+    return rc
+
+
+def ptr(x):
+    """Device pointer of a torch tensor, or an int / None passed through."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def stream_ptr(stream=None):
+    if stream is None:
+        try:
+            import torch
+            return torch.cuda.current_stream().cuda_stream
+        except Exception:  # pragma: no cover - torch without a device
+            return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+class Context:
+    """An hplai_ctx bound to one device (default: torch's current device)."""
+
+    def __init__(self, device=None):
+        if device is None:
+            try:
+                import torch
+                device = torch.cuda.current_device()
+            except Exception:
+                device = 0
+        h = _vp()
+        check(lib().hplai_create(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            lib().hplai_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- generator ----
+    def gen_matrix(self, n, seed, i0, j0, m, k, d, A, lda, stream=None):
+        check(lib().hplai_gen_matrix_f32(self.handle, n, seed, i0, j0, m, k, ptr(d), ptr(A), lda,
+                                         stream_ptr(stream)))
+
+    def gen_diag(self, n, seed, i0, cnt, d, stream=None):
+        check(lib().hplai_gen_diag_f64(self.handle, n, seed, i0, cnt, ptr(d), stream_ptr(stream)))
+
+    def gen_rhs(self, n, seed, i0, cnt, b, stream=None):
+        check(lib().hplai_gen_rhs_f64(self.handle, n, seed, i0, cnt, ptr(b), stream_ptr(stream)))
+
+    # ---- LU blocks ----
+    def getrf(self, n, A, lda, info=None, stream=None):
+        check(lib().hplai_getrf_nopiv_f32(self.handle, n, ptr(A), lda, ptr(info), stream_ptr(stream)))
+
+    def trsm_lpanel(self, m, b, U, ldu, L, ldl, stream=None):
+        check(lib().hplai_trsm_lpanel_f32(self.handle, m, b, ptr(U), ldu, ptr(L), ldl, stream_ptr(stream)))
+
+    def trsm_upanel(self, b, n, Lkk, ldl, U, ldu, stream=None):
+        check(lib().hplai_trsm_upanel_f32(self.handle, b, n, ptr(Lkk), ldl, ptr(U), ldu, stream_ptr(stream)))
+
+    def quantize(self, fmt, side, m, k, tb, src, lds, dst, ldd, exps=None, stream=None):
+        check(lib().hplai_quantize(self.handle, fmt, side, m, k, tb, ptr(src), lds, ptr(dst), ldd,
+                                   ptr(exps), stream_ptr(stream)))
+
+    def schur_update(self, fmt, m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu, tb, C,
+                     ldc, stream=None):
+        check(lib().hplai_schur_update(self.handle, fmt, m, n, nslices, kslice, ptr(Lq), ldl, ptr(eL),
+                                       ldel, ptr(Uq), ldu, ptr(eU), ldeu, tb, ptr(C), ldc,
+                                       stream_ptr(stream)))
+
+    # ---- IR ----
+    def otf_gemv(self, n, seed, d, alpha, x, y, stream=None):
+        check(lib().hplai_otf_gemv_f64(self.handle, n, seed, ptr(d), alpha, ptr(x), ptr(y),
+                                       stream_ptr(stream)))
+
+    def trsv_lower(self, n, bs, LU, lda, x, stream=None):
+        check(lib().hplai_trsv_lower_f32(self.handle, n, bs, ptr(LU), lda, ptr(x), stream_ptr(stream)))
+
+    def trsv_upper(self, n, bs, LU, lda, x, stream=None):
+        check(lib().hplai_trsv_upper_f32(self.handle, n, bs, ptr(LU), lda, ptr(x), stream_ptr(stream)))
+
+    def norm_inf(self, n, x, out, stream=None):
+        check(lib().hplai_norm_inf_f64(self.handle, n, ptr(x), ptr(out), stream_ptr(stream)))
+
+    # ---- whole solve ----
+    def solve(self, n, b, fmt="f16", outer=1, lookahead=True, lazy=False, seed=42, maxit=50,
+              verbose=False, keep_factors=False):
+        p = Params(n=n, b=b, seed=seed, fmt=FMTS[fmt] if isinstance(fmt, str) else fmt, outer=outer,
+                   lookahead=1 if lookahead else 0, lazy=1 if lazy else 0, maxit=maxit,
+                   verbose=1 if verbose else 0, keep_factors=1 if keep_factors else 0)
+        r = Result()
+        rc = lib().hplai_solve(self.handle, ctypes.byref(p), ctypes.byref(r))
+        if rc not in (OK, ENOCONV):
+            raise HplaiError(rc, last_error())
+        k = r.ir_steps
+        return dict(converged=(rc == OK), time_gen=r.time_gen, time_lu=r.time_lu, time_ir=r.time_ir,
+                    time_total=r.time_total, gflops=r.gflops, ir_steps=k, residual=r.residual,
+                    harness=r.harness, trace_resid=list(r.trace_resid[:k]),
+                    trace_harness=list(r.trace_harness[:k]), checksum=list(r.checksum))
+
+    def solution_x(self, n):
+        import numpy as np
+        x = np.zeros(n, dtype=np.float64)
+        check(lib().hplai_solution_x(self.handle, x.ctypes.data))
+        return x
+
+    def solution_factors(self, n):
+        import numpy as np
+        lu = np.zeros((n, n), dtype=np.float32)   # lu[j, i] = factor (i, j)
+        check(lib().hplai_solution_factors(self.handle, lu.ctypes.data, n))
+        return lu
+
+    def release(self):
+        check(lib().hplai_release(self.handle))
+
+
+def exported_symbols():
+    return [name for name, _, _ in _SIGS]

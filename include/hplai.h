@@ -1,0 +1,167 @@
+/*
+ * hplai.h — C ABI of the MI355X-native HPL-AI mixed-precision solver
+ * (libhplai_gfx950.so, built from hpl-ai_amd/csrc).
+ *
+ * Drop-in boundary for the hot path of RIKEN-RCCS/hpl-ai: each entry point
+ * names the reference interface it replaces (file:line in that tree).
+ *
+ * Conventions
+ *  - Matrices are column-major; leading dimensions are in elements (int64).
+ *  - Pointers marked "device" are HIP device pointers owned by the caller.
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream); calls
+ *    are stream-ordered and asynchronous unless stated otherwise.
+ *  - Every function returns HPLAI_OK (0) or a negative HPLAI_E* code; nothing
+ *    aborts or throws across the ABI.  hplai_last_error() gives the message
+ *    of the last failure on the calling thread.
+ *  - One hplai_ctx per host thread; a ctx binds one device.
+ */
+#ifndef HPLAI_H
+#define HPLAI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HPLAI_OK        0
+#define HPLAI_EINVAL   -1   /* bad argument (shape, pointer, mode)            */
+#define HPLAI_EHIP     -2   /* HIP runtime error                              */
+#define HPLAI_ENOMEM   -3   /* device allocation failed                       */
+#define HPLAI_ENOCONV  -4   /* IR did not reach harness < 16 within maxit     */
+#define HPLAI_EPIVOT   -5   /* zero or non-finite pivot in the diagonal LU    */
+
+/* Low-precision format of the Schur update operands */
+#define HPLAI_FMT_F16      0   /* binary16, exponents all zero (reference: no -d)       */
+#define HPLAI_FMT_F16_TILE 1   /* binary16 with per-b x b-tile power-of-two exponents   */
+#define HPLAI_FMT_E4M3     2   /* OCP FP8 e4m3 with per-b x b-tile power-of-two exps    */
+
+typedef struct hplai_ctx_s *hplai_ctx;
+
+int         hplai_create(int device, hplai_ctx *out);
+int         hplai_destroy(hplai_ctx ctx);
+const char *hplai_last_error(void);
+int         hplai_version(void);              /* MAJOR*10000 + MINOR*100 + PATCH */
+
+/* ---------------- generator: bit-exact with hpl_rand.hpp ----------------
+ * Element (i,j) of the n x n matrix is LCG state 1+i+n*j mapped to
+ * [-0.5,0.5) and rounded to FP32; the diagonal is the row's FP64 sum of
+ * |off-diagonals| (diagonal dominance).
+ */
+/* Replaces fill_one_panel_with_rand<float> (hpl_rand.hpp:59-106) as used by
+ * pmatgen (matgen.hpp:7-22).  Writes the m x k block at global (i0,j0) into A.
+ * d (device, length n, may be NULL): when given, global diagonal entries in
+ * the block become (float)d[i]; when NULL they are left as LCG values. */
+int hplai_gen_matrix_f32(hplai_ctx ctx, int64_t n, uint64_t seed, int64_t i0, int64_t j0,
+                         int64_t m, int64_t k, const double *d, float *A, int64_t lda,
+                         void *stream);
+/* Replaces calc_diag / pdiaggen (hpl_rand.hpp:108-123, matgen.hpp:209-225):
+ * d[t] = sum_{j != i} |a_ij| for i = i0+t, FP64, j ascending (bit-exact). */
+int hplai_gen_diag_f64(hplai_ctx ctx, int64_t n, uint64_t seed, int64_t i0, int64_t cnt,
+                       double *d, void *stream);
+/* Replaces pcolvgen (matgen.hpp:193-207): b[t] = value of state 1+(i0+t)+n*n. */
+int hplai_gen_rhs_f64(hplai_ctx ctx, int64_t n, uint64_t seed, int64_t i0, int64_t cnt,
+                      double *b, void *stream);
+
+/* ---------------- LU building blocks (FP32 panel) ---------------- */
+/* Replaces getrf_nopiv(n, a, lda, piv, ldpiv) (getrf_nopiv.hpp:143-160):
+ * unpivoted LU of the n x n block in place (unit-lower L, upper U), n <= 1024.
+ * info_dev (device int, may be NULL) receives 0, or k+1 for the first zero /
+ * non-finite pivot. */
+int hplai_getrf_nopiv_f32(hplai_ctx ctx, int n, float *A, int64_t lda, int *info_dev,
+                          void *stream);
+/* Replaces update_left_panels (panel_trf.hpp:143-157, strsm "R","U","N","N"):
+ * L(m x b) := L * U^-1, U the b x b upper triangle of Ukk. */
+int hplai_trsm_lpanel_f32(hplai_ctx ctx, int64_t m, int b, const float *Ukk, int64_t ldu,
+                          float *L, int64_t ldl, void *stream);
+/* Replaces update_right_panels (panel_trf.hpp:158-171, strsm "L","L","N","U"):
+ * U(b x n) := Lkk^-1 * U, Lkk unit lower. */
+int hplai_trsm_upanel_f32(hplai_ctx ctx, int b, int64_t n, const float *Lkk, int64_t ldl,
+                          float *U, int64_t ldu, void *stream);
+/* Replaces convert_left_panels / convert_right_panels (panel_trf.hpp:173-233)
+ * and pack_convert_{a,b}_opt (kernels/kernel.h:24-25).
+ * side 0 (L panel): src is m x k column-major; dst is written K-contiguous:
+ *   dst[i*ldd + kk] for row i, kk < k (row-major m x ldd).
+ * side 1 (U panel): src is k x m column-major (k rows); dst[j*ldd + kk].
+ * fmt: HPLAI_FMT_*; element size 2 (F16*) or 1 (E4M3) bytes.
+ * Per-tile exponents: tiles are tb x tb (tb = tile size, e.g. the block size
+ * b) indexed by (row or column)/tb relative to the panel start;
+ * exps[t] receives e with q = round(x * 2^e) (0 for HPLAI_FMT_F16).
+ * exps may be NULL only for HPLAI_FMT_F16.  Columns kk in [k, ldd) of dst
+ * are not touched. */
+int hplai_quantize(hplai_ctx ctx, int fmt, int side, int64_t m, int64_t k, int tb,
+                   const float *src, int64_t lds, void *dst, int64_t ldd, int32_t *exps,
+                   void *stream);
+/* Replaces gemmschur / GemmControl (schur_updator.hpp:9-47, 178-269) and
+ * hgemm/shgemm (fp16sim.hpp:197-233):
+ *   C(m x n) -= sum_s 2^-(eL[s][i/tb] + eU[s][j/tb]) * Lq(:, slice s) Uq(slice s, :)
+ * Lq is K-contiguous (Lq[i*ldl + kk]), Uq is K-contiguous (Uq[j*ldu + kk]);
+ * K = nslices * kslice; FP32 accumulation, C FP32.  eL/eU are
+ * [nslices][ntiles] int32 with row strides ldel/ldeu (NULL for HPLAI_FMT_F16);
+ * i and j count from the first row / column of C and tb must be a multiple
+ * of the kernel tile (128) when exponents are used. */
+int hplai_schur_update(hplai_ctx ctx, int fmt, int64_t m, int64_t n, int nslices, int kslice,
+                       const void *Lq, int64_t ldl, const int32_t *eL, int64_t ldel,
+                       const void *Uq, int64_t ldu, const int32_t *eU, int64_t ldeu, int tb,
+                       float *C, int64_t ldc, void *stream);
+
+/* ---------------- iterative refinement (FP64) ---------------- */
+/* Replaces panel_gemv / otf_gemv / otf_gemv_kernel (panel_gemv.hpp:57-180,
+ * otf_gemv.cpp:5-131): y := y + alpha * A x with A regenerated on the fly
+ * from (n, seed) and diagonal d (device, FP64). */
+int hplai_otf_gemv_f64(hplai_ctx ctx, int64_t n, uint64_t seed, const double *d, double alpha,
+                       const double *x, double *y, void *stream);
+/* Replaces panel_trsvL / tttrsvL (panel_trsv.hpp:115-124, 310-535):
+ * x := L^-1 x, L the unit lower triangle of LU (FP32), x FP64.  bs is the
+ * blocking (<= 1024; use the LU block size). */
+int hplai_trsv_lower_f32(hplai_ctx ctx, int64_t n, int bs, const float *LU, int64_t lda,
+                         double *x, void *stream);
+/* Replaces panel_trsvU / tttrsvU (panel_trsv.hpp:142-153, 537-758). */
+int hplai_trsv_upper_f32(hplai_ctx ctx, int64_t n, int bs, const float *LU, int64_t lda,
+                         double *x, void *stream);
+/* Replaces colv_infnorm (panel_norm.hpp:20-41): *out_dev = max_i |x_i|. */
+int hplai_norm_inf_f64(hplai_ctx ctx, int64_t n, const double *x, double *out_dev,
+                       void *stream);
+
+/* ---------------- whole solve (main.cpp:256-452) ---------------- */
+typedef struct hplai_params {
+    int64_t  n;          /* matrix order, n % b == 0                               */
+    int      b;          /* block size (reference `b`)                             */
+    uint64_t seed;       /* 42 in the reference (main.cpp:174)                     */
+    int      fmt;        /* HPLAI_FMT_*                                            */
+    int      outer;      /* panels per trailing update (1 = reference schedule)    */
+    int      lookahead;  /* 1: factor panel k+1 on a second stream during update k */
+    int      lazy;       /* 1: -d lazy initialisation (lazy_init.hpp:92-227): the trailing
+                            matrix starts at zero and accumulates -sum L U; each block gets
+                            its generated a_ij added when it joins a panel                */
+    int      maxit;      /* IR iterations (50 in the reference, 1 under -w)        */
+    int      verbose;    /* print the reference's IR trace lines                   */
+    int      keep_factors; /* leave A (factors) allocated for hplai_solution_*     */
+} hplai_params;
+
+typedef struct hplai_result {
+    double time_gen;     /* matrix + diag + rhs generation (not in time_total)     */
+    double time_lu;
+    double time_ir;
+    double time_total;   /* LU + IR, the reference's timed region (main.cpp:379-433) */
+    double gflops;       /* (2/3 n^3 + 3/2 n^2) / time_total * 1e-9 (main.cpp:441)  */
+    int    ir_steps;     /* residual evaluations                                   */
+    double residual;     /* final ||b - Ax||_inf (-1 if not converged)             */
+    double harness;      /* final HPL-AI harness (-1 if not converged)             */
+    double trace_resid[64];
+    double trace_harness[64];
+    double checksum[3];  /* panel_check fingerprint of the FP32 factors             */
+} hplai_result;
+
+/* Generate, factor and refine; synchronous.  Returns HPLAI_ENOCONV if the
+ * harness stays >= 16 (result still filled). */
+int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *r);
+/* After hplai_solve with keep_factors: copy the solution x (n doubles) and
+ * the factors (n x n FP32, ld n) to host memory. */
+int hplai_solution_x(hplai_ctx ctx, double *x_host);
+int hplai_solution_factors(hplai_ctx ctx, float *lu_host, int64_t ldh);
+int hplai_release(hplai_ctx ctx);   /* free buffers kept by keep_factors */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HPLAI_H */
