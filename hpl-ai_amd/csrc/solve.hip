@@ -75,6 +75,48 @@ double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+struct GemmTimer {
+    bool on = false;
+    std::vector<hipEvent_t> ev;   // start/stop pairs
+    std::vector<double> flops;
+    int rc = HPLAI_OK;
+    void begin(hipStream_t s, double f)
+    {
+        if (!on || rc) return;
+        hipEvent_t a = nullptr, b = nullptr;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            rc = HPLAI_EHIP;
+            return;
+        }
+        ev.push_back(a);
+        ev.push_back(b);
+        flops.push_back(f);
+        (void)hipEventRecord(a, s);
+    }
+    void end(hipStream_t s)
+    {
+        if (!on || rc || ev.empty()) return;
+        (void)hipEventRecord(ev.back(), s);
+    }
+    // after synchronisation
+    void collect(hplai_result *r)
+    {
+        double ms = 0.0, fl = 0.0;
+        for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+            float t = 0.0f;
+            if (hipEventElapsedTime(&t, ev[i], ev[i + 1]) == hipSuccess) ms += t;
+            fl += flops[i / 2];
+        }
+        r->schur_launches = static_cast<int64_t>(flops.size());
+        r->schur_flops = fl;
+        r->schur_ms = ms;
+    }
+    ~GemmTimer()
+    {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
 struct LU {
     int64_t n, lda, ldq;
     int b, outer, fmt, kpad, es;
@@ -85,6 +127,7 @@ struct LU {
     hipStream_t main, side;
     hipEvent_t *ev;
     int num_cu;
+    GemmTimer *timer;
 
     bool tiled() const { return fmt != HPLAI_FMT_F16; }
     float *a(int64_t i, int64_t j) const { return B->A + j * lda + i; }
@@ -117,9 +160,12 @@ struct LU {
              hipStream_t s) const
     {
         if (m <= 0 || nn <= 0) return HPLAI_OK;
-        return launch_schur(fmt, m, nn, nsl, kpad, lq(buf, row0, t0), ldq, el(buf, t0, row0), ntiles,
-                            uq(buf, col0, t0), ldq, eu(buf, t0, col0), ntiles, b, a(row0, col0), lda,
-                            num_cu, s);
+        timer->begin(s, 2.0 * static_cast<double>(m) * static_cast<double>(nn) * nsl * b);
+        const int rc = launch_schur(fmt, m, nn, nsl, kpad, lq(buf, row0, t0), ldq, el(buf, t0, row0),
+                                    ntiles, uq(buf, col0, t0), ldq, eu(buf, t0, col0), ntiles, b,
+                                    a(row0, col0), lda, num_cu, s);
+        timer->end(s);
+        return rc;
     }
 
     // Lazy initialisation (-d): add the generated a_ij to the partial sums of
@@ -286,6 +332,9 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
 
     LU lu;
     Buffers B;
+    GemmTimer timer;
+    timer.on = p->profile != 0;
+    lu.timer = &timer;
     lu.n = n;
     lu.lda = (n + 63) / 64 * 64;
     lu.b = b;
@@ -359,6 +408,8 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     HPLAI_HIP(hipDeviceSynchronize());
     double t2 = now_s();
     res->time_lu = t2 - t1;
+    if (rc == HPLAI_OK) rc = timer.rc;
+    timer.collect(res);
     if (rc == HPLAI_OK) {
         std::vector<int> info(lu.ntiles);
         HPLAI_HIP(hipMemcpy(info.data(), B.info, sizeof(int) * lu.ntiles, hipMemcpyDeviceToHost));

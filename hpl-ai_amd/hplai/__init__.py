@@ -30,13 +30,14 @@ class HplaiError(RuntimeError):
 class Params(ctypes.Structure):
     _fields_ = [("n", _i64), ("b", _int), ("seed", _u64), ("fmt", _int), ("outer", _int),
                 ("lookahead", _int), ("lazy", _int), ("maxit", _int), ("verbose", _int),
-                ("keep_factors", _int)]
+                ("keep_factors", _int), ("profile", _int)]
 
 
 class Result(ctypes.Structure):
     _fields_ = [("time_gen", _dbl), ("time_lu", _dbl), ("time_ir", _dbl), ("time_total", _dbl),
                 ("gflops", _dbl), ("ir_steps", _int), ("residual", _dbl), ("harness", _dbl),
-                ("trace_resid", _dbl * 64), ("trace_harness", _dbl * 64), ("checksum", _dbl * 3)]
+                ("trace_resid", _dbl * 64), ("trace_harness", _dbl * 64), ("checksum", _dbl * 3),
+                ("schur_launches", _i64), ("schur_flops", _dbl), ("schur_ms", _dbl)]
 
 
 # (name, restype, argtypes) for every exported entry point of include/hplai.h
@@ -193,10 +194,11 @@ class Context:
 
     # ---- whole solve ----
     def solve(self, n, b, fmt="f16", outer=1, lookahead=True, lazy=False, seed=42, maxit=50,
-              verbose=False, keep_factors=False):
+              verbose=False, keep_factors=False, profile=False):
         p = Params(n=n, b=b, seed=seed, fmt=FMTS[fmt] if isinstance(fmt, str) else fmt, outer=outer,
                    lookahead=1 if lookahead else 0, lazy=1 if lazy else 0, maxit=maxit,
-                   verbose=1 if verbose else 0, keep_factors=1 if keep_factors else 0)
+                   verbose=1 if verbose else 0, keep_factors=1 if keep_factors else 0,
+                   profile=1 if profile else 0)
         r = Result()
         rc = lib().hplai_solve(self.handle, ctypes.byref(p), ctypes.byref(r))
         if rc not in (OK, ENOCONV):
@@ -205,7 +207,8 @@ class Context:
         return dict(converged=(rc == OK), time_gen=r.time_gen, time_lu=r.time_lu, time_ir=r.time_ir,
                     time_total=r.time_total, gflops=r.gflops, ir_steps=k, residual=r.residual,
                     harness=r.harness, trace_resid=list(r.trace_resid[:k]),
-                    trace_harness=list(r.trace_harness[:k]), checksum=list(r.checksum))
+                    trace_harness=list(r.trace_harness[:k]), checksum=list(r.checksum),
+                    schur_launches=r.schur_launches, schur_flops=r.schur_flops, schur_ms=r.schur_ms)
 
     def solution_x(self, n):
         import numpy as np

@@ -136,6 +136,7 @@ typedef struct hplai_params {
     int      maxit;      /* IR iterations (50 in the reference, 1 under -w)        */
     int      verbose;    /* print the reference's IR trace lines                   */
     int      keep_factors; /* leave A (factors) allocated for hplai_solution_*     */
+    int      profile;    /* time every Schur-update launch with HIP events         */
 } hplai_params;
 
 typedef struct hplai_result {
@@ -150,6 +151,11 @@ typedef struct hplai_result {
     double trace_resid[64];
     double trace_harness[64];
     double checksum[3];  /* panel_check fingerprint of the FP32 factors             */
+    /* with profile: Schur-update launches, their algorithmic flops (2 m n K each)
+       and summed event-measured durations */
+    int64_t schur_launches;
+    double  schur_flops;
+    double  schur_ms;
 } hplai_result;
 
 /* Generate, factor and refine; synchronous.  Returns HPLAI_ENOCONV if the

@@ -4,7 +4,12 @@ Tolerances:
   generator / quantizer  : bit-exact (integer and rounding work)
   FP32 panel (getrf/trsm): max |gpu - oracle| / max |oracle| < 2e-5
   Schur MFMA update      : per element |err| <= 4 K 2^-24 (|Lq||Uq|)_ij + 2^-23 |C_ij|
-                           against an FP64 product of the same quantized operands
+                           against an FP64 product of the same quantized operands;
+                           FP8 adds 2^-10 (|Lq||Uq|)_ij: v_mfma_scale_f32_32x32x64_f8f6f4
+                           sums the products of one internal dot group in a fixed-point
+                           window 13 bits below the group's largest product (measured:
+                           2^16 + 2^3 kept, 2^16 + 2^2 dropped within a group; exact
+                           FP32 accumulation across groups and instructions)
   FP64 residual / trsv   : relative 1e-12 (summation order differs)
   whole solve            : HPL-AI harness < 16 (iterative_refinement.hpp:43-48);
                            factors within 1e-4 of the oracle at n=1200 b=60
@@ -236,6 +241,8 @@ def test_schur_update_vs_fp64(ctx, oracle, fmt, m, n, nsl, tb):
         Pabs += sc * (np.abs(Ls) @ np.abs(Us.T))
     want = C0.T.astype(np.float64) - P
     tol = 4 * K * 2.0 ** -24 * Pabs + 2.0 ** -23 * np.abs(C0.T) + 1e-30
+    if fmt == "e4m3":
+        tol += 2.0 ** -10 * Pabs
     err = np.abs(got.T - want)
     bad = err > tol
     assert not bad.any(), "max err/tol %.3g at %s" % ((err / tol).max(), np.argwhere(bad)[:3])
