@@ -130,7 +130,7 @@ int hplai_trsm_lpanel_f32(hplai_ctx ctx, int64_t m, int b, const float *Ukk, int
     CTX_CHECK(ctx);
     HPLAI_CHECK(m >= 0 && b >= 0 && b <= 1024, "trsm_lpanel: bad shape");
     HPLAI_CHECK(Ukk != nullptr && L != nullptr && ldu >= b && ldl >= m, "trsm_lpanel: bad pointers/ld");
-    return launch_trsm_l(m, b, Ukk, ldu, L, ldl, nullptr, 1, as_stream(stream));
+    return launch_trsm_l(m, b, Ukk, ldu, L, ldl, nullptr, 1, nullptr, as_stream(stream));
 }
 
 int hplai_trsm_upanel_f32(hplai_ctx ctx, int b, int64_t n, const float *Lkk, int64_t ldl, float *U,
@@ -139,7 +139,7 @@ int hplai_trsm_upanel_f32(hplai_ctx ctx, int b, int64_t n, const float *Lkk, int
     CTX_CHECK(ctx);
     HPLAI_CHECK(n >= 0 && b >= 0 && b <= 1024, "trsm_upanel: bad shape");
     HPLAI_CHECK(Lkk != nullptr && U != nullptr && ldl >= b && ldu >= b, "trsm_upanel: bad pointers/ld");
-    return launch_trsm_u(b, n, Lkk, ldl, U, ldu, nullptr, 1, as_stream(stream));
+    return launch_trsm_u(b, n, Lkk, ldl, U, ldu, nullptr, 1, nullptr, as_stream(stream));
 }
 
 int hplai_quantize(hplai_ctx ctx, int fmt, int side, int64_t m, int64_t k, int tb, const float *src,

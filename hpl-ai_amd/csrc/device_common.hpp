@@ -111,4 +111,58 @@ constexpr int kF16Er = 15;
 
 __device__ __forceinline__ float ldexp_fast(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
 
+// atomic max of a non-negative float, ordered as its uint32 bit pattern
+__device__ __forceinline__ void atomic_max_pos(float *p, float v)
+{
+    atomicMax(reinterpret_cast<unsigned int *>(p), __float_as_uint(v));
+}
+
+// Max of v over the lanes of the wave that share `tile`; one atomic per tile.
+// Every lane of the wave must call it (inactive lanes pass active = false).
+__device__ __forceinline__ void tile_max_reduce(float *amax, int64_t tile, float v, bool active)
+{
+    if (amax == nullptr) return;
+    const int64_t t0 = __shfl(tile, 0);
+    const bool uniform = __all(!active || tile == t0);
+    if (uniform) {
+        float m = active ? v : 0.0f;
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        if (__lane_id() == 0 && __any(active)) atomic_max_pos(amax + t0, m);
+    } else if (active) {
+        atomic_max_pos(amax + tile, v);
+    }
+}
+
+}  // namespace hplai
+
+namespace hplai {
+
+// In-place inverse of a lower-triangular 32 x 32 matrix held in LDS
+// (M[r * 33 + c], non-zero diagonal) by recursive doubling:
+//   [A 0; C B]^-1 = [A^-1 0; -B^-1 C A^-1, B^-1]
+// with block sizes 1, 2, 4, 8, 16.  T is a 32 x 33 LDS scratch.  Every thread
+// of the workgroup must call it (it contains barriers).
+__device__ inline void lower_inv32(float *M, float *T, int tid, int nthr)
+{
+    for (int i = tid; i < 32; i += nthr) M[i * 33 + i] = 1.0f / M[i * 33 + i];
+    __syncthreads();
+    for (int s = 1; s < 32; s *= 2) {
+        const int np = 16 / s, per = s * s;
+        for (int e = tid; e < np * per; e += nthr) {      // T = C A^-1
+            const int p = e / per, ij = e % per, i = ij / s, j = ij % s, base = p * 2 * s;
+            float acc = 0.0f;
+            for (int k = j; k < s; ++k) acc = fmaf(M[(base + s + i) * 33 + base + k], M[(base + k) * 33 + base + j], acc);
+            T[(base + s + i) * 33 + base + j] = acc;
+        }
+        __syncthreads();
+        for (int e = tid; e < np * per; e += nthr) {      // C = -B^-1 T
+            const int p = e / per, ij = e % per, i = ij / s, j = ij % s, base = p * 2 * s;
+            float acc = 0.0f;
+            for (int k = 0; k <= i; ++k) acc = fmaf(M[(base + s + i) * 33 + base + s + k], T[(base + s + k) * 33 + base + j], acc);
+            M[(base + s + i) * 33 + base + j] = -acc;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace hplai

@@ -47,10 +47,15 @@ int launch_gen_diag(int64_t n, uint64_t seed, int64_t i0, int64_t cnt, double *d
 int launch_gen_rhs(int64_t n, uint64_t seed, int64_t i0, int64_t cnt, double *b, hipStream_t s);
 
 int launch_getrf(int n, float *A, int64_t lda, int *info, hipStream_t s);
+// Diagonal-block sub-inverses used by the strip TRSMs: for each 32 x 32
+// diagonal sub-block of a factored b x b block, Linv then Uinv (row-major).
+int64_t diag_inv_floats(int b);
+int launch_diag_inv(int b, const float *LU, int64_t lda, float *inv, hipStream_t s);
+// inv: the factored diagonal block's sub-inverses, or NULL (computed on the fly)
 int launch_trsm_l(int64_t m, int b, const float *U, int64_t ldu, float *L, int64_t ldl,
-                  float *amax, int tb, hipStream_t s);
+                  float *amax, int tb, const float *inv, hipStream_t s);
 int launch_trsm_u(int b, int64_t n, const float *Lkk, int64_t ldl, float *U, int64_t ldu,
-                  float *amax, int tb, hipStream_t s);
+                  float *amax, int tb, const float *inv, hipStream_t s);
 // amax: per-tile maxima of |src| (nullable for FMT_F16 → exponents 0).
 int launch_quantize(int fmt, int side, int64_t m, int64_t k, int tb, const float *src, int64_t lds,
                     void *dst, int64_t ldd, const float *amax, int32_t *exps, hipStream_t s);

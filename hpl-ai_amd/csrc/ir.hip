@@ -103,19 +103,30 @@ __global__ __launch_bounds__(256) void trsv_diag_lower_kernel(int bs, const floa
     for (int sb = 0; sb < bs; sb += 32) {
         const int w = bs - sb < 32 ? bs - sb : 32;
         if (wave == 0) {
+            // lane i holds x_i and row i of the 32 x 32 sub-block (loaded up front)
             double xr = lane < w ? xs[sb + lane] : 0.0;
-            for (int c = 0; c < w; ++c) {
-                const double xc = __shfl(xr, c);
-                if (lane > c && lane < w)
-                    xr = xr - xc * static_cast<double>(Lkk[static_cast<int64_t>(sb + c) * lda + sb + lane]);
+            float lrow[32];
+#pragma unroll
+            for (int c = 0; c < 32; ++c)
+                lrow[c] = (lane < w && c < w) ? Lkk[static_cast<int64_t>(sb + c) * lda + sb + lane] : 0.0f;
+#pragma unroll
+            for (int c = 0; c < 32; ++c) {
+                if (c < w) {
+                    const double xc = __shfl(xr, c);
+                    if (lane > c && lane < w) xr = xr - xc * static_cast<double>(lrow[c]);
+                }
             }
             if (lane < w) xs[sb + lane] = xr;
         }
         __syncthreads();
         for (int r = sb + w + tid; r < bs; r += 256) {
+            float lv[32];   // all 32 loads in flight at once
+#pragma unroll
+            for (int c = 0; c < 32; ++c) lv[c] = c < w ? Lkk[static_cast<int64_t>(sb + c) * lda + r] : 0.0f;
             double acc = 0.0;
-            for (int c = 0; c < w; ++c)
-                acc = fma(static_cast<double>(Lkk[static_cast<int64_t>(sb + c) * lda + r]), xs[sb + c], acc);
+#pragma unroll
+            for (int c = 0; c < 32; ++c)
+                if (c < w) acc = fma(static_cast<double>(lv[c]), xs[sb + c], acc);
             xs[r] -= acc;
         }
         __syncthreads();
@@ -136,20 +147,29 @@ __global__ __launch_bounds__(256) void trsv_diag_upper_kernel(int bs, const floa
         const int w = bs - sb < 32 ? bs - sb : 32;
         if (wave == 0) {
             double xr = lane < w ? xs[sb + lane] : 0.0;
-            for (int c = w - 1; c >= 0; --c) {
-                if (lane == c)
-                    xr = xr / static_cast<double>(Ukk[static_cast<int64_t>(sb + c) * lda + sb + c]);
-                const double xc = __shfl(xr, c);
-                if (lane < c)
-                    xr = xr - xc * static_cast<double>(Ukk[static_cast<int64_t>(sb + c) * lda + sb + lane]);
+            float urow[32];
+#pragma unroll
+            for (int c = 0; c < 32; ++c)
+                urow[c] = (lane < w && c < w) ? Ukk[static_cast<int64_t>(sb + c) * lda + sb + lane] : 0.0f;
+#pragma unroll
+            for (int c = 31; c >= 0; --c) {
+                if (c < w) {
+                    if (lane == c) xr = xr / static_cast<double>(urow[c]);
+                    const double xc = __shfl(xr, c);
+                    if (lane < c) xr = xr - xc * static_cast<double>(urow[c]);
+                }
             }
             if (lane < w) xs[sb + lane] = xr;
         }
         __syncthreads();
         for (int r = tid; r < sb; r += 256) {
+            float uv[32];
+#pragma unroll
+            for (int c = 0; c < 32; ++c) uv[c] = c < w ? Ukk[static_cast<int64_t>(sb + c) * lda + r] : 0.0f;
             double acc = 0.0;
-            for (int c = 0; c < w; ++c)
-                acc = fma(static_cast<double>(Ukk[static_cast<int64_t>(sb + c) * lda + r]), xs[sb + c], acc);
+#pragma unroll
+            for (int c = 0; c < 32; ++c)
+                if (c < w) acc = fma(static_cast<double>(uv[c]), xs[sb + c], acc);
             xs[r] -= acc;
         }
         __syncthreads();
@@ -157,24 +177,41 @@ __global__ __launch_bounds__(256) void trsv_diag_upper_kernel(int bs, const floa
     for (int r = tid; r < bs; r += 256) x[r] = xs[r];
 }
 
-// y(0:m) -= P(m x w) * xb(0:w), P column-major (ttgemv, panel_trsv.hpp:10-45)
+// y(0:m) -= P(m x w) * xb(0:w), P column-major (ttgemv, panel_trsv.hpp:10-45).
+// Workgroup = 64 rows x 4 column groups; the 4 partial sums of a row are added
+// in a fixed order through LDS.
+constexpr int kGemvRowsT = 64;
+
 __global__ __launch_bounds__(256) void trsv_gemv_kernel(int64_t m, int w, const float *__restrict__ P,
                                                         int64_t lda, const double *__restrict__ xb,
                                                         double *__restrict__ y)
 {
     __shared__ double xs[kTrsvMax];
+    __shared__ double part[4][kGemvRowsT];
     for (int c = threadIdx.x; c < w; c += 256) xs[c] = xb[c];
     __syncthreads();
-    const int64_t r = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (r >= m) return;
+    const int lr = threadIdx.x % kGemvRowsT, g = threadIdx.x / kGemvRowsT;
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * kGemvRowsT + lr;
+    const int cw = (w + 3) / 4, c0 = g * cw, c1 = c0 + cw < w ? c0 + cw : w;
     double acc0 = 0.0, acc1 = 0.0;
-    int c = 0;
-    for (; c + 1 < w; c += 2) {
-        acc0 = fma(static_cast<double>(P[static_cast<int64_t>(c) * lda + r]), xs[c], acc0);
-        acc1 = fma(static_cast<double>(P[static_cast<int64_t>(c + 1) * lda + r]), xs[c + 1], acc1);
+    if (r < m) {
+        const float *pr = P + r;
+        int c = c0;
+        for (; c + 7 < c1; c += 8) {
+            float v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = pr[static_cast<int64_t>(c + q) * lda];
+#pragma unroll
+            for (int q = 0; q < 8; q += 2) {
+                acc0 = fma(static_cast<double>(v[q]), xs[c + q], acc0);
+                acc1 = fma(static_cast<double>(v[q + 1]), xs[c + q + 1], acc1);
+            }
+        }
+        for (; c < c1; ++c) acc0 = fma(static_cast<double>(pr[static_cast<int64_t>(c) * lda]), xs[c], acc0);
     }
-    if (c < w) acc0 = fma(static_cast<double>(P[static_cast<int64_t>(c) * lda + r]), xs[c], acc0);
-    y[r] -= acc0 + acc1;
+    part[g][lr] = acc0 + acc1;
+    __syncthreads();
+    if (g == 0 && r < m) y[r] -= ((part[0][lr] + part[1][lr]) + part[2][lr]) + part[3][lr];
 }
 
 int launch_trsv_lower(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s)
@@ -186,7 +223,7 @@ int launch_trsv_lower(int64_t n, int bs, const float *LU, int64_t lda, double *x
         HPLAI_HIP(hipGetLastError());
         const int64_t m = n - kb - w;
         if (m > 0) {
-            hipLaunchKernelGGL(trsv_gemv_kernel, dim3(static_cast<unsigned>(ceil_div(m, 256))),
+            hipLaunchKernelGGL(trsv_gemv_kernel, dim3(static_cast<unsigned>(ceil_div(m, kGemvRowsT))),
                                dim3(256), 0, s, m, w, Lkk + w, lda, x + kb, x + kb + w);
             HPLAI_HIP(hipGetLastError());
         }
@@ -204,7 +241,7 @@ int launch_trsv_upper(int64_t n, int bs, const float *LU, int64_t lda, double *x
         hipLaunchKernelGGL(trsv_diag_upper_kernel, dim3(1), dim3(256), 0, s, w, Ukk, lda, x + kb);
         HPLAI_HIP(hipGetLastError());
         if (kb > 0) {
-            hipLaunchKernelGGL(trsv_gemv_kernel, dim3(static_cast<unsigned>(ceil_div(kb, 256))),
+            hipLaunchKernelGGL(trsv_gemv_kernel, dim3(static_cast<unsigned>(ceil_div(kb, kGemvRowsT))),
                                dim3(256), 0, s, kb, w, LU + kb * lda, lda, x + kb, x);
             HPLAI_HIP(hipGetLastError());
         }

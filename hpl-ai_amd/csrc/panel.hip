@@ -7,29 +7,6 @@
 namespace hplai {
 
 // ---------------------------------------------------------------------------
-// atomic max of a non-negative float, ordered as its uint32 bit pattern
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void atomic_max_pos(float *p, float v)
-{
-    atomicMax(reinterpret_cast<unsigned int *>(p), __float_as_uint(v));
-}
-
-// Max of v over the lanes of the wave that share `tile`; one atomic per tile.
-__device__ __forceinline__ void tile_max_reduce(float *amax, int64_t tile, float v, bool active)
-{
-    if (amax == nullptr) return;
-    const int64_t t0 = __shfl(tile, 0);
-    const bool uniform = __all(!active || tile == t0);
-    if (uniform) {
-        float m = active ? v : 0.0f;
-        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-        if (__lane_id() == 0 && __any(active)) atomic_max_pos(amax + t0, m);
-    } else if (active) {
-        atomic_max_pos(amax + tile, v);
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Diagonal-block LU (getrf_nopiv, getrf_nopiv.hpp:124-141 with NSMALL panels)
 //
 // One workgroup of 512 threads factors an n x n block (n <= NMAX) in place.
@@ -43,84 +20,142 @@ template <int NMAX, int W>
 __global__ __launch_bounds__(kGetrfThreads) void getrf_kernel(int n, float *__restrict__ A,
                                                               int64_t lda, int *__restrict__ info)
 {
-    __shared__ float Lc[W * NMAX];   // panel, Lc[c * NMAX + r]
-    __shared__ float Us[W * NMAX];   // U strip, Us[r * NMAX + c]
-    const int tid = threadIdx.x;
     constexpr int NT = kGetrfThreads;
+    constexpr int RPT = NMAX / NT;   // panel rows per thread
+    // +8: the 8 x 8 tiles of the trailing update may read past the last row /
+    // column (values unused)
+    __shared__ __attribute__((aligned(16))) float Lc[W * NMAX + 8];   // panel, Lc[c * NMAX + r]
+    __shared__ __attribute__((aligned(16))) float Us[W * NMAX + 8];   // U strip, Us[r * NMAX + c]
+    __shared__ float prow[2][W];     // pivot row, double-buffered
+    __shared__ float Mi[32 * 33], Ti[32 * 33];   // L11 inverse (W == 32)
+    const int tid = threadIdx.x;
     int bad = 0;
     for (int k = 0; k < n; k += W) {
         const int w = n - k < W ? n - k : W;
         const int h = n - k;
         float *Ak = A + static_cast<int64_t>(k) * lda + k;
-        for (int idx = tid; idx < h * w; idx += NT) {
-            int r = idx % h, c = idx / h;
-            Lc[c * NMAX + r] = Ak[static_cast<int64_t>(c) * lda + r];
+        // each thread holds its panel rows in registers
+        float p[RPT][W];
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = tid + q * NT;
+#pragma unroll
+            for (int c = 0; c < W; ++c)
+                p[q][c] = (r < h && c < w) ? Ak[static_cast<int64_t>(c) * lda + r] : 0.0f;
+        }
+        // unblocked right-looking LU of the h x w panel (getrf_nopiv_small,
+        // getrf_nopiv.hpp:30-46, continued down the rows = trsmR); the owner of
+        // row kk publishes it, one barrier per pivot
+#pragma unroll
+        for (int kk = 0; kk < W; ++kk) {
+            if (kk < w) {
+                if (tid == kk)
+#pragma unroll
+                    for (int c = 0; c < W; ++c) prow[kk & 1][c] = p[0][c];
+                __syncthreads();
+                const float piv = prow[kk & 1][kk];
+                const float inv = 1.0f / piv;
+                if (tid == 0 && bad == 0 && !(piv != 0.0f && isfinite(piv))) bad = k + kk + 1;
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    const int r = tid + q * NT;
+                    if (r > kk && r < h) {
+                        const float l = p[q][kk] * inv;
+                        p[q][kk] = l;
+#pragma unroll
+                        for (int c = kk + 1; c < W; ++c) p[q][c] = p[q][c] - l * prow[kk & 1][c];
+                    }
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = tid + q * NT;
+            if (r < h)
+#pragma unroll
+                for (int c = 0; c < W; ++c)
+                    if (c < w) {
+                        Ak[static_cast<int64_t>(c) * lda + r] = p[q][c];
+                        Lc[c * NMAX + r] = p[q][c];
+                    }
         }
         __syncthreads();
-        // unblocked right-looking LU of the h x w panel (getrf_nopiv_small,
-        // getrf_nopiv.hpp:30-46, continued down the rows = trsmR)
-        for (int kk = 0; kk < w; ++kk) {
-            const float piv = Lc[kk * NMAX + kk];
-            const float inv = 1.0f / piv;
-            if (tid == 0 && bad == 0 && !(piv != 0.0f && isfinite(piv))) bad = k + kk + 1;
-            for (int r = kk + 1 + tid; r < h; r += NT) {
-                float l = Lc[kk * NMAX + r] * inv;
-                Lc[kk * NMAX + r] = l;
-                for (int c = kk + 1; c < w; ++c)
-                    Lc[c * NMAX + r] = Lc[c * NMAX + r] - l * Lc[c * NMAX + kk];
+        const int wr = n - k - w;   // columns right of the panel
+        if constexpr (W == 32) {
+            // U strip through the inverse of the unit-lower L11: x := L11^-1 x
+            for (int e = tid; e < 32 * 32; e += NT) {
+                const int r = e / 32, c = e % 32;
+                Mi[r * 33 + c] = (r > c && r < w) ? Lc[c * NMAX + r] : (r == c ? 1.0f : 0.0f);
             }
             __syncthreads();
-        }
-        for (int idx = tid; idx < h * w; idx += NT) {
-            int r = idx % h, c = idx / h;
-            Ak[static_cast<int64_t>(c) * lda + r] = Lc[c * NMAX + r];
-        }
-        const int wr = n - k - w;   // columns right of the panel
-        // U strip: x := L11^-1 x per column (strsm "L","L","N","U" order)
-        for (int c = tid; c < wr; c += NT) {
-            float x[W];
-            float *col = Ak + static_cast<int64_t>(w + c) * lda;
+            lower_inv32(Mi, Ti, tid, NT);
+            for (int c = tid; c < wr; c += NT) {
+                float x[W];
+                float *col = Ak + static_cast<int64_t>(w + c) * lda;
 #pragma unroll
-            for (int r = 0; r < W; ++r) x[r] = r < w ? col[r] : 0.0f;
+                for (int r = 0; r < W; ++r) x[r] = r < w ? col[r] : 0.0f;
+                for (int i = 0; i < w; ++i) {
+                    float acc = 0.0f;
 #pragma unroll
-            for (int r = 0; r < W; ++r)
-#pragma unroll
-                for (int i = r + 1; i < W; ++i) x[i] = x[i] - x[r] * Lc[r * NMAX + i];
-#pragma unroll
-            for (int r = 0; r < W; ++r)
-                if (r < w) {
-                    col[r] = x[r];
-                    Us[r * NMAX + c] = x[r];
+                    for (int r = 0; r < W; ++r) acc = fmaf(Mi[i * 33 + r], x[r], acc);
+                    col[i] = acc;
+                    Us[i * NMAX + c] = acc;
                 }
+            }
+        } else {
+            // U strip: x := L11^-1 x per column (strsm "L","L","N","U" order)
+            for (int c = tid; c < wr; c += NT) {
+                float x[W];
+                float *col = Ak + static_cast<int64_t>(w + c) * lda;
+#pragma unroll
+                for (int r = 0; r < W; ++r) x[r] = r < w ? col[r] : 0.0f;
+#pragma unroll
+                for (int r = 0; r < W; ++r)
+#pragma unroll
+                    for (int i = r + 1; i < W; ++i) x[i] = x[i] - x[r] * Lc[r * NMAX + i];
+#pragma unroll
+                for (int r = 0; r < W; ++r)
+                    if (r < w) {
+                        col[r] = x[r];
+                        Us[r * NMAX + c] = x[r];
+                    }
+            }
         }
         __syncthreads();
-        // trailing update, 4x4 register tiles; rows map fastest for coalescing
+        // trailing update A22 -= L21 U12, 8 x 8 register tiles (rows fastest
+        // across threads); L21 and U12 come from LDS as 16-byte reads.  Rows /
+        // columns past the edge read padding and are never stored.
         const int hr = h - w;
         if (hr > 0 && wr > 0) {
-            const int rt = (hr + 3) / 4, ct = (wr + 3) / 4;
-            for (int t = tid; t < rt * ct; t += NT) {
-                const int r0 = w + 4 * (t % rt), c0 = 4 * (t / rt);
-                float acc[4][4] = {};
+            const int RT = (hr + 7) / 8, CT = (wr + 7) / 8;
+            for (int t = tid; t < RT * CT; t += NT) {
+                const int r0 = w + 8 * (t % RT), c0 = 8 * (t / RT);
+                float acc[8][8];   // [column][row], starts from A22
+#pragma unroll
+                for (int bq = 0; bq < 8; ++bq)
+#pragma unroll
+                    for (int a = 0; a < 8; ++a)
+                        acc[bq][a] = (r0 + a < h && c0 + bq < wr)
+                                         ? Ak[static_cast<int64_t>(w + c0 + bq) * lda + r0 + a] : 0.0f;
                 for (int kk = 0; kk < w; ++kk) {
-                    float l[4], u[4];
+                    const float4 l0 = *reinterpret_cast<const float4 *>(&Lc[kk * NMAX + r0]);
+                    const float4 l1 = *reinterpret_cast<const float4 *>(&Lc[kk * NMAX + r0 + 4]);
+                    const float4 u0 = *reinterpret_cast<const float4 *>(&Us[kk * NMAX + c0]);
+                    const float4 u1 = *reinterpret_cast<const float4 *>(&Us[kk * NMAX + c0 + 4]);
+                    const float l[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+                    const float u[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        l[q] = Lc[kk * NMAX + (r0 + q < h ? r0 + q : h - 1)];
-                        u[q] = Us[kk * NMAX + (c0 + q < wr ? c0 + q : wr - 1)];
-                    }
+                    for (int bq = 0; bq < 8; ++bq)
 #pragma unroll
-                    for (int a = 0; a < 4; ++a)
-#pragma unroll
-                        for (int b = 0; b < 4; ++b) acc[b][a] = fmaf(l[a], u[b], acc[b][a]);
+                        for (int a = 0; a < 8; ++a) acc[bq][a] = fmaf(-l[a], u[bq], acc[bq][a]);
                 }
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    if (c0 + b >= wr) break;
-                    float *col = Ak + static_cast<int64_t>(w + c0 + b) * lda;
+                for (int bq = 0; bq < 8; ++bq)
 #pragma unroll
-                    for (int a = 0; a < 4; ++a)
-                        if (r0 + a < h) col[r0 + a] = col[r0 + a] - acc[b][a];
-                }
+                    for (int a = 0; a < 8; ++a)
+                        if (r0 + a < h && c0 + bq < wr)
+                            Ak[static_cast<int64_t>(w + c0 + bq) * lda + r0 + a] = acc[bq][a];
             }
         }
         __syncthreads();
@@ -360,10 +395,34 @@ int launch_getrf(int n, float *A, int64_t lda, int *info, hipStream_t s)
     return HPLAI_OK;
 }
 
+bool trsm_strip_ok(int b, const float *amax, int tb);
+int launch_trsm_l_strip(int64_t m, int b, const float *U, int64_t ldu, const float *inv, float *L,
+                        int64_t ldl, float *amax, int tb, hipStream_t s);
+int launch_trsm_u_strip(int b, int64_t n, const float *Lkk, int64_t ldl, const float *inv, float *U,
+                        int64_t ldu, float *amax, int tb, hipStream_t s);
+
+// Strip TRSMs need the diagonal sub-block inverses; compute them into a
+// temporary when the caller did not supply them.
+template <typename F>
+static int with_inv(int b, const float *LU, int64_t ld, const float *inv, hipStream_t s, F &&f)
+{
+    if (inv != nullptr) return f(inv);
+    float *tmp = nullptr;
+    HPLAI_HIP(hipMallocAsync(reinterpret_cast<void **>(&tmp), sizeof(float) * diag_inv_floats(b), s));
+    int rc = launch_diag_inv(b, LU, ld, tmp, s);
+    if (rc == HPLAI_OK) rc = f(tmp);
+    HPLAI_HIP(hipFreeAsync(tmp, s));
+    return rc;
+}
+
 int launch_trsm_l(int64_t m, int b, const float *U, int64_t ldu, float *L, int64_t ldl, float *amax,
-                  int tb, hipStream_t s)
+                  int tb, const float *inv, hipStream_t s)
 {
     if (m <= 0 || b <= 0) return HPLAI_OK;
+    if (trsm_strip_ok(b, amax, tb))
+        return with_inv(b, U, ldu, inv, s, [&](const float *iv) {
+            return launch_trsm_l_strip(m, b, U, ldu, iv, L, ldl, amax, tb, s);
+        });
     hipLaunchKernelGGL(trsm_l_kernel, dim3(static_cast<unsigned>(ceil_div(m, 256))), dim3(256), 0, s,
                        m, b, U, ldu, L, ldl, amax, tb);
     HPLAI_HIP(hipGetLastError());
@@ -371,9 +430,13 @@ int launch_trsm_l(int64_t m, int b, const float *U, int64_t ldu, float *L, int64
 }
 
 int launch_trsm_u(int b, int64_t n, const float *Lkk, int64_t ldl, float *U, int64_t ldu,
-                  float *amax, int tb, hipStream_t s)
+                  float *amax, int tb, const float *inv, hipStream_t s)
 {
     if (n <= 0 || b <= 0) return HPLAI_OK;
+    if (trsm_strip_ok(b, amax, tb))
+        return with_inv(b, Lkk, ldl, inv, s, [&](const float *iv) {
+            return launch_trsm_u_strip(b, n, Lkk, ldl, iv, U, ldu, amax, tb, s);
+        });
     hipLaunchKernelGGL(trsm_u_kernel, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s,
                        b, n, Lkk, ldl, U, ldu, amax, tb);
     HPLAI_HIP(hipGetLastError());

@@ -42,6 +42,7 @@ struct Buffers {
     float *amaxL[2] = {nullptr, nullptr}, *amaxU[2] = {nullptr, nullptr};
     int *info = nullptr;
     double *chk = nullptr;
+    float *inv = nullptr;   // diagonal sub-block inverses, one set per diagonal block
 };
 
 void free_buffers(Buffers &B, bool keep_factors)
@@ -60,6 +61,7 @@ void free_buffers(Buffers &B, bool keep_factors)
     f(B.norms);
     f(B.info);
     f(B.chk);
+    f(B.inv);
     for (int i = 0; i < 2; ++i) {
         f(B.Lq[i]);
         f(B.Uq[i]);
@@ -155,6 +157,7 @@ struct LU {
     {
         return tiled() ? B->amaxU[buf] + t * ntiles + col / b : nullptr;
     }
+    const float *dinv(int64_t c) const { return B->inv + (c / b) * diag_inv_floats(b); }
 
     int gemm(int buf, int t0, int nsl, int64_t row0, int64_t col0, int64_t m, int64_t nn,
              hipStream_t s) const
@@ -192,7 +195,10 @@ struct LU {
             if (rc == HPLAI_OK) rc = launch_getrf(b, a(c, c), lda, B->info + c / b, s);
             const int64_t mrow = n - c - b;
             if (rc || mrow <= 0) continue;
-            rc = launch_trsm_l(mrow, b, a(c, c), lda, a(c + b, c), lda, aml(buf, t, c + b), b, s);
+            if (b <= 512) rc = launch_diag_inv(b, a(c, c), lda, const_cast<float *>(dinv(c)), s);
+            if (rc == HPLAI_OK)
+                rc = launch_trsm_l(mrow, b, a(c, c), lda, a(c + b, c), lda, aml(buf, t, c + b), b,
+                                   b <= 512 ? dinv(c) : nullptr, s);
             if (rc == HPLAI_OK)
                 rc = launch_quantize(fmt, 0, mrow, b, b, a(c + b, c), lda, lq(buf, c + b, t), ldq,
                                      aml(buf, t, c + b), el(buf, t, c + b), s);
@@ -200,7 +206,8 @@ struct LU {
             if (rc || ncols <= 0) continue;
             rc = lazy_init(c, c + b, b, ncols, s);   // block row inside the panel
             if (rc == HPLAI_OK)
-                rc = launch_trsm_u(b, ncols, a(c, c), lda, a(c, c + b), lda, amu(buf, t, c + b), b, s);
+                rc = launch_trsm_u(b, ncols, a(c, c), lda, a(c, c + b), lda, amu(buf, t, c + b), b,
+                                   b <= 512 ? dinv(c) : nullptr, s);
             if (rc == HPLAI_OK)
                 rc = launch_quantize(fmt, 1, ncols, b, b, a(c, c + b), lda, uq(buf, c + b, t), ldq,
                                      amu(buf, t, c + b), eu(buf, t, c + b), s);
@@ -220,7 +227,8 @@ struct LU {
             const int64_t c = c0 + static_cast<int64_t>(t) * b;
             rc = lazy_init(c, col0, b, ncols, s);   // block row right of the outer panel
             if (rc == HPLAI_OK)
-                rc = launch_trsm_u(b, ncols, a(c, c), lda, a(c, col0), lda, amu(buf, t, col0), b, s);
+                rc = launch_trsm_u(b, ncols, a(c, c), lda, a(c, col0), lda, amu(buf, t, col0), b,
+                                   b <= 512 ? dinv(c) : nullptr, s);
             if (rc == HPLAI_OK)
                 rc = launch_quantize(fmt, 1, ncols, b, b, a(c, col0), lda, uq(buf, col0, t), ldq,
                                      amu(buf, t, col0), eu(buf, t, col0), s);
@@ -372,6 +380,7 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     ALLOC(B.norms, sizeof(double) * 4);
     ALLOC(B.info, sizeof(int) * lu.ntiles);
     ALLOC(B.chk, sizeof(double) * 3);
+    ALLOC(B.inv, sizeof(float) * diag_inv_floats(b) * lu.ntiles);
     for (int i = 0; i < 2; ++i) {
         const size_t qbytes = static_cast<size_t>(n) * lu.ldq * es;
         ALLOC(B.Lq[i], qbytes);
