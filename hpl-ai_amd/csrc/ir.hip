@@ -25,7 +25,13 @@ __global__ __launch_bounds__(kGemvRows) void otf_gemv_partial_kernel(
     const int64_t cb = blockIdx.x / row_blocks;
     const int64_t j0 = cb * kGemvChunk;
     const int64_t j1 = j0 + kGemvChunk < n ? j0 + kGemvChunk : n;
-    for (int64_t t = threadIdx.x; t < j1 - j0; t += kGemvRows) xs[t] = x[j0 + t];
+    {
+        double v[kGemvChunk / kGemvRows];   // all loads in flight together
+#pragma unroll
+        for (int u = 0; u < kGemvChunk / kGemvRows; ++u) v[u] = x[min(j0 + u * kGemvRows + threadIdx.x, j1 - 1)];
+#pragma unroll
+        for (int u = 0; u < kGemvChunk / kGemvRows; ++u) xs[u * kGemvRows + threadIdx.x] = v[u];
+    }
     __syncthreads();
     const int64_t i = rb * kGemvRows + threadIdx.x;
     if (i >= n) return;
@@ -108,7 +114,10 @@ __global__ __launch_bounds__(256) void trsv_diag_lower_kernel(int bs, const floa
             float lrow[32];
 #pragma unroll
             for (int c = 0; c < 32; ++c)
-                lrow[c] = (lane < w && c < w) ? Lkk[static_cast<int64_t>(sb + c) * lda + sb + lane] : 0.0f;
+            {
+                const float v = Lkk[static_cast<int64_t>(sb + min(c, w - 1)) * lda + sb + min(lane, w - 1)];
+                lrow[c] = (lane < w && c < w) ? v : 0.0f;
+            }
 #pragma unroll
             for (int c = 0; c < 32; ++c) {
                 if (c < w) {
@@ -122,7 +131,10 @@ __global__ __launch_bounds__(256) void trsv_diag_lower_kernel(int bs, const floa
         for (int r = sb + w + tid; r < bs; r += 256) {
             float lv[32];   // all 32 loads in flight at once
 #pragma unroll
-            for (int c = 0; c < 32; ++c) lv[c] = c < w ? Lkk[static_cast<int64_t>(sb + c) * lda + r] : 0.0f;
+            for (int c = 0; c < 32; ++c) {
+                const float v = Lkk[static_cast<int64_t>(sb + min(c, w - 1)) * lda + r];
+                lv[c] = c < w ? v : 0.0f;
+            }
             double acc = 0.0;
 #pragma unroll
             for (int c = 0; c < 32; ++c)
@@ -150,7 +162,10 @@ __global__ __launch_bounds__(256) void trsv_diag_upper_kernel(int bs, const floa
             float urow[32];
 #pragma unroll
             for (int c = 0; c < 32; ++c)
-                urow[c] = (lane < w && c < w) ? Ukk[static_cast<int64_t>(sb + c) * lda + sb + lane] : 0.0f;
+            {
+                const float v = Ukk[static_cast<int64_t>(sb + min(c, w - 1)) * lda + sb + min(lane, w - 1)];
+                urow[c] = (lane < w && c < w) ? v : 0.0f;
+            }
 #pragma unroll
             for (int c = 31; c >= 0; --c) {
                 if (c < w) {
@@ -165,7 +180,10 @@ __global__ __launch_bounds__(256) void trsv_diag_upper_kernel(int bs, const floa
         for (int r = tid; r < sb; r += 256) {
             float uv[32];
 #pragma unroll
-            for (int c = 0; c < 32; ++c) uv[c] = c < w ? Ukk[static_cast<int64_t>(sb + c) * lda + r] : 0.0f;
+            for (int c = 0; c < 32; ++c) {
+                const float v = Ukk[static_cast<int64_t>(sb + min(c, w - 1)) * lda + r];
+                uv[c] = c < w ? v : 0.0f;
+            }
             double acc = 0.0;
 #pragma unroll
             for (int c = 0; c < 32; ++c)

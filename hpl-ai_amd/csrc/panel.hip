@@ -40,8 +40,10 @@ __global__ __launch_bounds__(kGetrfThreads) void getrf_kernel(int n, float *__re
         for (int q = 0; q < RPT; ++q) {
             const int r = tid + q * NT;
 #pragma unroll
-            for (int c = 0; c < W; ++c)
-                p[q][c] = (r < h && c < w) ? Ak[static_cast<int64_t>(c) * lda + r] : 0.0f;
+            for (int c = 0; c < W; ++c) {
+                const float v = Ak[static_cast<int64_t>(min(c, w - 1)) * lda + min(r, h - 1)];
+                p[q][c] = (r < h && c < w) ? v : 0.0f;
+            }
         }
         // unblocked right-looking LU of the h x w panel (getrf_nopiv_small,
         // getrf_nopiv.hpp:30-46, continued down the rows = trsmR); the owner of
@@ -94,7 +96,10 @@ __global__ __launch_bounds__(kGetrfThreads) void getrf_kernel(int n, float *__re
                 float x[W];
                 float *col = Ak + static_cast<int64_t>(w + c) * lda;
 #pragma unroll
-                for (int r = 0; r < W; ++r) x[r] = r < w ? col[r] : 0.0f;
+                for (int r = 0; r < W; ++r) {
+                    const float v = col[min(r, w - 1)];
+                    x[r] = r < w ? v : 0.0f;
+                }
                 for (int i = 0; i < w; ++i) {
                     float acc = 0.0f;
 #pragma unroll
@@ -109,7 +114,10 @@ __global__ __launch_bounds__(kGetrfThreads) void getrf_kernel(int n, float *__re
                 float x[W];
                 float *col = Ak + static_cast<int64_t>(w + c) * lda;
 #pragma unroll
-                for (int r = 0; r < W; ++r) x[r] = r < w ? col[r] : 0.0f;
+                for (int r = 0; r < W; ++r) {
+                    const float v = col[min(r, w - 1)];
+                    x[r] = r < w ? v : 0.0f;
+                }
 #pragma unroll
                 for (int r = 0; r < W; ++r)
 #pragma unroll
@@ -136,8 +144,7 @@ __global__ __launch_bounds__(kGetrfThreads) void getrf_kernel(int n, float *__re
                 for (int bq = 0; bq < 8; ++bq)
 #pragma unroll
                     for (int a = 0; a < 8; ++a)
-                        acc[bq][a] = (r0 + a < h && c0 + bq < wr)
-                                         ? Ak[static_cast<int64_t>(w + c0 + bq) * lda + r0 + a] : 0.0f;
+                        acc[bq][a] = Ak[static_cast<int64_t>(w + min(c0 + bq, wr - 1)) * lda + min(r0 + a, h - 1)];
                 for (int kk = 0; kk < w; ++kk) {
                     const float4 l0 = *reinterpret_cast<const float4 *>(&Lc[kk * NMAX + r0]);
                     const float4 l1 = *reinterpret_cast<const float4 *>(&Lc[kk * NMAX + r0 + 4]);

@@ -13,9 +13,12 @@
 // E4M3 : v_mfma_scale_f32_32x32x64_f8f6f4 with the tile exponents as E8M0 block
 //        scales, so the products come out unscaled.
 //
-// Workgroup: 256 threads (4 waves, 2 x 2), tile 128 x 128, each wave 64 x 64 as
-// 2 x 2 MFMA blocks of 32 x 32.  128 bytes of K per row per stage for both
-// formats (64 FP16 / 128 FP8), double-buffered in LDS with a 16-byte row pad.
+// Workgroup: 256 threads (4 waves, 2 x 2), tile BM x BN (256 x 256 for the
+// large updates, 128 x 128 otherwise); each wave computes (BM/2) x (BN/2) as
+// (BM/64) x (BN/64) MFMA blocks of 32 x 32.  128 bytes of K per row per stage
+// for both formats (64 FP16 / 128 FP8), double-buffered in LDS with a 16-byte
+// row pad.  A 256 x 256 tile reads 8 KB of LDS per 16 MFMAs per wave, which
+// keeps LDS traffic below the MFMA time.
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -26,7 +29,7 @@ typedef float floatx16_t __attribute__((ext_vector_type(16)));
 typedef int intx8_t __attribute__((ext_vector_type(8)));
 typedef int intx4_t __attribute__((ext_vector_type(4)));
 
-constexpr int kBM = 128, kBN = 128, kThreads = 256;
+constexpr int kThreads = 256;
 constexpr int kRowBytes = 128;               // K bytes per row per stage
 constexpr int kLdsRow = kRowBytes + 16;      // padded LDS row (bank spread)
 
@@ -45,7 +48,7 @@ __device__ __forceinline__ int xcd_remap(int w, int nwg)
     return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
 
-template <int FMT, bool VEC_C>
+template <int FMT, int BM, int BN, bool VEC_C>
 __global__ __launch_bounds__(kThreads) void schur_kernel(
     int64_t M, int64_t N, int nslices, int kslice, const uint8_t *__restrict__ Lq, int64_t ldl_b,
     const int32_t *__restrict__ eL, int64_t ldel, const uint8_t *__restrict__ Uq, int64_t ldu_b,
@@ -53,8 +56,10 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
     int tiles_m, int tiles_n)
 {
     using F = SchurFmt<FMT>;
-    __shared__ __attribute__((aligned(16))) uint8_t As[2][kBM * kLdsRow];
-    __shared__ __attribute__((aligned(16))) uint8_t Bs[2][kBN * kLdsRow];
+    constexpr int IB = BM / 64, JB = BN / 64;          // 32 x 32 blocks per wave
+    constexpr int QA = BM * 8 / kThreads, QB = BN * 8 / kThreads;   // 16-byte chunks per thread
+    __shared__ __attribute__((aligned(16))) uint8_t As[2][BM * kLdsRow];
+    __shared__ __attribute__((aligned(16))) uint8_t Bs[2][BN * kLdsRow];
 
     // ---- tile coordinates: XCD remap, then groups of 8 tile-rows ----
     const int nwg = tiles_m * tiles_n;
@@ -66,47 +71,59 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
     const int in_group = logical % per_group;
     const int tm = first_m + in_group % gsize;
     const int tn = in_group / gsize;
-    const int64_t m0 = static_cast<int64_t>(tm) * kBM;
-    const int64_t n0 = static_cast<int64_t>(tn) * kBN;
+    const int64_t m0 = static_cast<int64_t>(tm) * BM;
+    const int64_t n0 = static_cast<int64_t>(tn) * BN;
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int wm = wave & 1, wn = wave >> 1;
     const int lrow = lane & 31, lgrp = lane >> 5;
 
-    // ---- global -> register staging: 4 x 16 B per operand per thread ----
-    intx4_t ra[4], rb[4];
+    // ---- global -> register staging: QA / QB x 16 B per thread ----
+    intx4_t ra[QA], rb[QB];
     const int64_t kbytes_total = static_cast<int64_t>(nslices) * kslice * F::es;
     const int nstages = static_cast<int>(kbytes_total / kRowBytes);
 
     auto gload = [&](int stage) {
         const int64_t kb = static_cast<int64_t>(stage) * kRowBytes;
+        // rows past the edge re-read the last valid row: they feed only
+        // accumulator rows / columns the epilogue never stores, so no select
+        // is needed and the loads' wait stays at the LDS store after the MFMAs
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < QA; ++q) {
             const int c = tid + kThreads * q;
             const int row = c >> 3, part = c & 7;
-            const int64_t gm = m0 + row, gn = n0 + row;
-            ra[q] = gm < M ? *reinterpret_cast<const intx4_t *>(Lq + gm * ldl_b + kb + part * 16)
-                           : intx4_t{0, 0, 0, 0};
-            rb[q] = gn < N ? *reinterpret_cast<const intx4_t *>(Uq + gn * ldu_b + kb + part * 16)
-                           : intx4_t{0, 0, 0, 0};
+            const int64_t gm = m0 + row;
+            ra[q] = *reinterpret_cast<const intx4_t *>(Lq + (gm < M ? gm : M - 1) * ldl_b + kb + part * 16);
+        }
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            const int c = tid + kThreads * q;
+            const int row = c >> 3, part = c & 7;
+            const int64_t gn = n0 + row;
+            rb[q] = *reinterpret_cast<const intx4_t *>(Uq + (gn < N ? gn : N - 1) * ldu_b + kb + part * 16);
         }
     };
     auto sstore = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < QA; ++q) {
             const int c = tid + kThreads * q;
             const int row = c >> 3, part = c & 7;
             *reinterpret_cast<intx4_t *>(&As[buf][row * kLdsRow + part * 16]) = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            const int c = tid + kThreads * q;
+            const int row = c >> 3, part = c & 7;
             *reinterpret_cast<intx4_t *>(&Bs[buf][row * kLdsRow + part * 16]) = rb[q];
         }
     };
 
-    floatx16_t acc[2][2];
+    floatx16_t acc[IB][JB];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < IB; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < JB; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
@@ -121,7 +138,9 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
     __syncthreads();
     for (int ks = 0; ks < nstages; ++ks) {
         const int buf = ks & 1;
-        if (ks + 1 < nstages) gload(ks + 1);
+        // the last stage re-loads itself and stores into the idle buffer, so
+        // the loop body is one basic block the scheduler can interleave
+        gload(ks + 1 < nstages ? ks + 1 : ks);
         if (ks % stages_per_slice == 0 && eL != nullptr) {
             const int s = ks / stages_per_slice;
             const int el = eL[s * ldel + row_tile], eu = eU[s * ldeu + col_tile];
@@ -135,9 +154,9 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
                 } else if (e_new != e_cur) {
                     const int d = e_new - e_cur;
 #pragma unroll
-                    for (int i = 0; i < 2; ++i)
+                    for (int i = 0; i < IB; ++i)
 #pragma unroll
-                        for (int j = 0; j < 2; ++j)
+                        for (int j = 0; j < JB; ++j)
 #pragma unroll
                             for (int r = 0; r < 16; ++r) acc[i][j][r] = ldexp_fast(acc[i][j][r], d);
                     e_cur = e_new;
@@ -149,63 +168,67 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
         if (FMT == HPLAI_FMT_E4M3) {
 #pragma unroll
             for (int kk = 0; kk < 2; ++kk) {
-                intx8_t a[2], b[2];
+                intx8_t a[IB], b[JB];
 #pragma unroll
-                for (int i = 0; i < 2; ++i) {
-                    const uint8_t *p = Ab + (wm * 64 + i * 32 + lrow) * kLdsRow + kk * 64 + lgrp * 32;
+                for (int i = 0; i < IB; ++i) {
+                    const uint8_t *p = Ab + (wm * (BM / 2) + i * 32 + lrow) * kLdsRow + kk * 64 + lgrp * 32;
                     intx4_t lo = *reinterpret_cast<const intx4_t *>(p);
                     intx4_t hi = *reinterpret_cast<const intx4_t *>(p + 16);
                     a[i] = intx8_t{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
                 }
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    const uint8_t *p = Bb + (wn * 64 + j * 32 + lrow) * kLdsRow + kk * 64 + lgrp * 32;
+                for (int j = 0; j < JB; ++j) {
+                    const uint8_t *p = Bb + (wn * (BN / 2) + j * 32 + lrow) * kLdsRow + kk * 64 + lgrp * 32;
                     intx4_t lo = *reinterpret_cast<const intx4_t *>(p);
                     intx4_t hi = *reinterpret_cast<const intx4_t *>(p + 16);
                     b[j] = intx8_t{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
                 }
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < IB; ++i)
 #pragma unroll
-                    for (int j = 0; j < 2; ++j)
+                    for (int j = 0; j < JB; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
                             a[i], b[j], acc[i][j], 0, 0, 0, scale_a, 0, scale_b);
             }
         } else {
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) {
-                half8_t a[2], b[2];
+                half8_t a[IB], b[JB];
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < IB; ++i)
                     a[i] = *reinterpret_cast<const half8_t *>(
-                        Ab + (wm * 64 + i * 32 + lrow) * kLdsRow + kk * 32 + lgrp * 16);
+                        Ab + (wm * (BM / 2) + i * 32 + lrow) * kLdsRow + kk * 32 + lgrp * 16);
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < JB; ++j)
                     b[j] = *reinterpret_cast<const half8_t *>(
-                        Bb + (wn * 64 + j * 32 + lrow) * kLdsRow + kk * 32 + lgrp * 16);
+                        Bb + (wn * (BN / 2) + j * 32 + lrow) * kLdsRow + kk * 32 + lgrp * 16);
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < IB; ++i)
 #pragma unroll
-                    for (int j = 0; j < 2; ++j)
+                    for (int j = 0; j < JB; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i], b[j], acc[i][j], 0, 0, 0);
             }
         }
-        if (ks + 1 < nstages) sstore(buf ^ 1);
+        // keep the prefetch's LDS stores (and their wait on the global loads)
+        // after this stage's MFMAs; buf ^ 1 was last read before the previous
+        // barrier, so storing into it is safe on every stage
+        __builtin_amdgcn_sched_barrier(0);
+        sstore(buf ^ 1);
         __syncthreads();
     }
 
     // ---- epilogue: C -= acc (FP16 tile mode: acc * 2^-e_cur) ----
     // acc[i][j][4*q + r] holds row 8q + 4*lgrp + r, column lrow of block (i, j)
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < IB; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int64_t gc = n0 + wn * 64 + j * 32 + lrow;
+        for (int j = 0; j < JB; ++j) {
+            const int64_t gc = n0 + wn * (BN / 2) + j * 32 + lrow;
             if (gc >= N) continue;
             float *ccol = C + gc * ldc;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int64_t gr = m0 + wm * 64 + i * 32 + 8 * q + 4 * lgrp;
+                const int64_t gr = m0 + wm * (BM / 2) + i * 32 + 8 * q + 4 * lgrp;
                 float v[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
@@ -228,14 +251,201 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
         }
 }
 
+// ---------------------------------------------------------------------------
+// Large-update kernel: 256 x 256 tile, operands streamed into LDS by the
+// direct global->LDS loads (global_load_lds_dwordx4), four LDS stages of 64 K
+// bytes per row, three of them in flight while the fourth feeds the MFMAs.
+// One stage is 32 KB (256 rows of Lq + 256 rows of Uq); a wave-wide load
+// fills 16 rows (1 KB) with lane l writing bytes [16 l, 16 l + 16).  The
+// 16-byte chunks of a row are XOR-swizzled (chunk c of row r sits at
+// position c ^ ((r >> 2) & 3)) so the 32 rows an MFMA operand read touches
+// fall in distinct banks; the swizzle is applied by choosing which global
+// chunk each lane fetches.  The waits are explicit: the compiler neither
+// orders nor counts the LDS side of these loads.
+// ---------------------------------------------------------------------------
+constexpr int kDmaTile = 256;
+constexpr int kDmaStageBytes = 64;                                  // K bytes per row per stage
+constexpr int kDmaStages = 4;
+constexpr int kDmaStage = 2 * kDmaTile * kDmaStageBytes;           // 32 KB
+constexpr int kDmaLoadsPerStage = 2 * kDmaTile * kDmaStageBytes / (kThreads * 16);   // 8 per thread
+
+template <int FMT, bool VEC_C>
+__global__ __launch_bounds__(kThreads) void schur_dma_kernel(
+    int64_t M, int64_t N, int nslices, int kslice, const uint8_t *__restrict__ Lq, int64_t ldl_b,
+    const int32_t *__restrict__ eL, int64_t ldel, const uint8_t *__restrict__ Uq, int64_t ldu_b,
+    const int32_t *__restrict__ eU, int64_t ldeu, int tb, float *__restrict__ C, int64_t ldc,
+    int tiles_m, int tiles_n)
+{
+    static_assert(FMT != HPLAI_FMT_F16_TILE, "per-tile FP16 rescales accumulators; not supported here");
+    static_assert(kDmaLoadsPerStage == 8, "four Lq and four Uq loads per thread per stage");
+    using F = SchurFmt<FMT>;
+    constexpr int BM = kDmaTile, BN = kDmaTile, SB = kDmaStageBytes;
+    __shared__ __attribute__((aligned(1024))) uint8_t S[kDmaStages * kDmaStage];
+
+    const int nwg = tiles_m * tiles_n;
+    const int logical = xcd_remap(static_cast<int>(blockIdx.x), nwg);
+    constexpr int G = 8;
+    const int per_group = G * tiles_n;
+    const int first_m = (logical / per_group) * G;
+    const int gsize = tiles_m - first_m < G ? tiles_m - first_m : G;
+    const int in_group = logical % per_group;
+    const int tm = first_m + in_group % gsize;
+    const int tn = in_group / gsize;
+    const int64_t m0 = static_cast<int64_t>(tm) * BM;
+    const int64_t n0 = static_cast<int64_t>(tn) * BN;
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int lrow = lane & 31, lgrp = lane >> 5;
+
+    // ---- per-thread sources: load q of this wave fills rows (4 q + wave) * 16 + lane / 4 ----
+    // rows past the edge re-read the last valid row; they only feed
+    // accumulator rows / columns the epilogue never stores
+    const uint8_t *srcA[4], *srcB[4];
+    {
+        const int chunk = (lane & 3) ^ ((lane >> 4) & 3);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = (4 * q + wave) * 16 + (lane >> 2);
+            const int64_t gm = m0 + row < M ? m0 + row : M - 1;
+            const int64_t gn = n0 + row < N ? n0 + row : N - 1;
+            srcA[q] = Lq + gm * ldl_b + chunk * 16;
+            srcB[q] = Uq + gn * ldu_b + chunk * 16;
+        }
+    }
+    const int nstages = static_cast<int>(static_cast<int64_t>(nslices) * kslice * F::es / SB);
+    const int stages_per_slice = kslice * F::es / SB;
+
+    // stage `st` of K into LDS buffer `buf`
+    auto issue = [&](int st, int buf) {
+        const int64_t kb = static_cast<int64_t>(st) * SB;
+        uint8_t *dst = S + buf * kDmaStage + wave * 1024;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            __builtin_amdgcn_global_load_lds(srcA[q] + kb, dst + q * 4096, 16, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            __builtin_amdgcn_global_load_lds(srcB[q] + kb, dst + BM * SB + q * 4096, 16, 0, 0);
+    };
+
+    floatx16_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    const int row_tile = tb > 0 ? static_cast<int>(m0 / tb) : 0;
+    const int col_tile = tb > 0 ? static_cast<int>(n0 / tb) : 0;
+    int scale_a = 127, scale_b = 127;
+
+    // operand read offsets inside a stage (swizzled chunk positions)
+    const int f = (lrow >> 2) & 3;
+    const int a_base = (wm * (BM / 2) + lrow) * SB;
+    const int b_base = BM * SB + (wn * (BN / 2) + lrow) * SB;
+
+    // prologue: stages 0..2 (clamped; extra copies land in buffers nobody reads)
+#pragma unroll
+    for (int st = 0; st < kDmaStages - 1; ++st) issue(st < nstages ? st : nstages - 1, st);
+
+    for (int ks = 0; ks < nstages; ++ks) {
+        const int buf = ks & (kDmaStages - 1);
+        // stage ks has landed once at most the two later stages' loads remain;
+        // the barrier publishes every wave's part and retires all reads of the
+        // buffer the next load overwrites
+        asm volatile("s_waitcnt vmcnt(16)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        {
+            const int nxt = ks + kDmaStages - 1;
+            issue(nxt < nstages ? nxt : nstages - 1, nxt & (kDmaStages - 1));
+        }
+        if (FMT == HPLAI_FMT_E4M3 && ks % stages_per_slice == 0) {
+            const int sl = ks / stages_per_slice;
+            scale_a = 127 - eL[sl * ldel + row_tile];
+            scale_b = 127 - eU[sl * ldeu + col_tile];
+        }
+        const uint8_t *St = S + buf * kDmaStage;
+        if (FMT == HPLAI_FMT_E4M3) {
+            const int off_lo = 16 * ((2 * lgrp) ^ f), off_hi = 16 * ((2 * lgrp + 1) ^ f);
+            intx8_t a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint8_t *p = St + a_base + i * 32 * SB;
+                const intx4_t lo = *reinterpret_cast<const intx4_t *>(p + off_lo);
+                const intx4_t hi = *reinterpret_cast<const intx4_t *>(p + off_hi);
+                a[i] = intx8_t{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint8_t *p = St + b_base + j * 32 * SB;
+                const intx4_t lo = *reinterpret_cast<const intx4_t *>(p + off_lo);
+                const intx4_t hi = *reinterpret_cast<const intx4_t *>(p + off_hi);
+                b[j] = intx8_t{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                        a[i], b[j], acc[i][j], 0, 0, 0, scale_a, 0, scale_b);
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk) {
+                const int off = 16 * ((2 * kk + lgrp) ^ f);
+                half8_t a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    a[i] = *reinterpret_cast<const half8_t *>(St + a_base + i * 32 * SB + off);
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    b[j] = *reinterpret_cast<const half8_t *>(St + b_base + j * 32 * SB + off);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+            }
+        }
+    }
+    // drain the clamped tail loads before the workgroup can exit
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+
+    // ---- epilogue: C -= acc ----
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t gc = n0 + wn * (BN / 2) + j * 32 + lrow;
+            if (gc >= N) continue;
+            float *ccol = C + gc * ldc;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int64_t gr = m0 + wm * (BM / 2) + i * 32 + 8 * q + 4 * lgrp;
+                if (VEC_C && gr + 3 < M) {
+                    float4 *p = reinterpret_cast<float4 *>(ccol + gr);
+                    float4 c4 = *p;
+                    c4.x -= acc[i][j][4 * q + 0];
+                    c4.y -= acc[i][j][4 * q + 1];
+                    c4.z -= acc[i][j][4 * q + 2];
+                    c4.w -= acc[i][j][4 * q + 3];
+                    *p = c4;
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (gr + r < M) ccol[gr + r] -= acc[i][j][4 * q + r];
+                }
+            }
+        }
+}
+
 template <int FMT>
-static int schur_dispatch(int64_t m, int64_t n, int nslices, int kslice, const void *Lq,
-                          int64_t ldl, const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
-                          const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc,
-                          hipStream_t s)
+static int schur_dma_launch(int64_t m, int64_t n, int nslices, int kslice, const void *Lq, int64_t ldl,
+                            const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
+                            const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, hipStream_t s)
 {
     using F = SchurFmt<FMT>;
-    const int64_t tiles_m = ceil_div(m, kBM), tiles_n = ceil_div(n, kBN);
+    const int64_t tiles_m = ceil_div(m, kDmaTile), tiles_n = ceil_div(n, kDmaTile);
     if (tiles_m * tiles_n >= (int64_t(1) << 31)) {
         set_error("schur_update: %lld x %lld tiles exceed one launch", (long long)tiles_m,
                   (long long)tiles_n);
@@ -246,15 +456,64 @@ static int schur_dispatch(int64_t m, int64_t n, int nslices, int kslice, const v
     const uint8_t *L8 = static_cast<const uint8_t *>(Lq);
     const uint8_t *U8 = static_cast<const uint8_t *>(Uq);
     if (vec)
-        hipLaunchKernelGGL((schur_kernel<FMT, true>), grid, dim3(kThreads), 0, s, m, n, nslices,
+        hipLaunchKernelGGL((schur_dma_kernel<FMT, true>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
+                           L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
+                           static_cast<int>(tiles_m), static_cast<int>(tiles_n));
+    else
+        hipLaunchKernelGGL((schur_dma_kernel<FMT, false>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
+                           L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
+                           static_cast<int>(tiles_m), static_cast<int>(tiles_n));
+    HPLAI_HIP(hipGetLastError());
+    return HPLAI_OK;
+}
+
+template <int FMT, int BM, int BN>
+static int schur_launch(int64_t m, int64_t n, int nslices, int kslice, const void *Lq, int64_t ldl,
+                        const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
+                        const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, hipStream_t s)
+{
+    using F = SchurFmt<FMT>;
+    const int64_t tiles_m = ceil_div(m, BM), tiles_n = ceil_div(n, BN);
+    if (tiles_m * tiles_n >= (int64_t(1) << 31)) {
+        set_error("schur_update: %lld x %lld tiles exceed one launch", (long long)tiles_m,
+                  (long long)tiles_n);
+        return HPLAI_EINVAL;
+    }
+    const bool vec = (reinterpret_cast<uintptr_t>(C) % 16 == 0) && (ldc % 4 == 0);
+    const dim3 grid(static_cast<unsigned>(tiles_m * tiles_n));
+    const uint8_t *L8 = static_cast<const uint8_t *>(Lq);
+    const uint8_t *U8 = static_cast<const uint8_t *>(Uq);
+    if (vec)
+        hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, true>), grid, dim3(kThreads), 0, s, m, n, nslices,
                            kslice, L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
                            static_cast<int>(tiles_m), static_cast<int>(tiles_n));
     else
-        hipLaunchKernelGGL((schur_kernel<FMT, false>), grid, dim3(kThreads), 0, s, m, n, nslices,
+        hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, false>), grid, dim3(kThreads), 0, s, m, n, nslices,
                            kslice, L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
                            static_cast<int>(tiles_m), static_cast<int>(tiles_n));
     HPLAI_HIP(hipGetLastError());
     return HPLAI_OK;
+}
+
+// 256 x 256 tiles for updates that fill the device; 128 x 128 for thin ones.
+// With per-tile exponents every workgroup tile must lie inside one tb x tb
+// tile, so the large tiles also need tb % 256 == 0.  The per-tile FP16 form
+// rescales its accumulators between slices and keeps the 128 x 128 tile
+// (the 256 x 256 accumulators live in AGPRs and would have to be copied out).
+template <int FMT>
+static int schur_dispatch(int64_t m, int64_t n, int nslices, int kslice, const void *Lq, int64_t ldl,
+                          const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
+                          const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, int num_cu,
+                          hipStream_t s)
+{
+    if constexpr (FMT != HPLAI_FMT_F16_TILE) {
+        const bool tiles_ok = (FMT == HPLAI_FMT_F16) || (tb % kDmaTile == 0);
+        if (tiles_ok && ceil_div(m, kDmaTile) * ceil_div(n, kDmaTile) >= 2 * num_cu)
+            return schur_dma_launch<FMT>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu,
+                                         tb, C, ldc, s);
+    }
+    return schur_launch<FMT, 128, 128>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu, tb,
+                                       C, ldc, s);
 }
 
 // Validation lives here so the C ABI and the solver share it.
@@ -263,8 +522,8 @@ int launch_schur(int fmt, int64_t m, int64_t n, int nslices, int kslice, const v
                  const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, int num_cu,
                  hipStream_t s)
 {
-    (void)num_cu;
     if (m <= 0 || n <= 0 || nslices <= 0) return HPLAI_OK;
+    if (num_cu <= 0) num_cu = 256;
     const int es = fmt == HPLAI_FMT_E4M3 ? 1 : 2;
     const int bk = kRowBytes / es;
     HPLAI_CHECK(fmt == HPLAI_FMT_F16 || fmt == HPLAI_FMT_F16_TILE || fmt == HPLAI_FMT_E4M3,
@@ -281,18 +540,18 @@ int launch_schur(int fmt, int64_t m, int64_t n, int nslices, int kslice, const v
     HPLAI_CHECK(ldc >= m, "schur_update: ldc < m");
     if (fmt != HPLAI_FMT_F16) {
         HPLAI_CHECK(eL != nullptr && eU != nullptr, "schur_update: exponent arrays required");
-        HPLAI_CHECK(tb > 0 && tb % kBM == 0, "schur_update: tile %d must be a multiple of %d", tb, kBM);
+        HPLAI_CHECK(tb > 0 && tb % 128 == 0, "schur_update: tile %d must be a multiple of 128", tb);
     }
     switch (fmt) {
     case HPLAI_FMT_F16:
         return schur_dispatch<HPLAI_FMT_F16>(m, n, nslices, kslice, Lq, ldl, nullptr, 0, Uq, ldu,
-                                             nullptr, 0, 0, C, ldc, s);
+                                             nullptr, 0, 0, C, ldc, num_cu, s);
     case HPLAI_FMT_F16_TILE:
         return schur_dispatch<HPLAI_FMT_F16_TILE>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu,
-                                                  eU, ldeu, tb, C, ldc, s);
+                                                  eU, ldeu, tb, C, ldc, num_cu, s);
     default:
         return schur_dispatch<HPLAI_FMT_E4M3>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU,
-                                              ldeu, tb, C, ldc, s);
+                                              ldeu, tb, C, ldc, num_cu, s);
     }
 }
 

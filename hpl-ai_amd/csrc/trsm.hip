@@ -27,10 +27,13 @@ __global__ __launch_bounds__(256) void diag_inv32_kernel(int b, const float *__r
     const int tid = threadIdx.x;
     const int sb = blockIdx.x * 32;
     const int w = b - sb < 32 ? b - sb : 32;
-    for (int e = tid; e < 32 * 32; e += 256) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int e = u * 256 + tid;
         const int r = e / 32, c = e % 32;
         const bool in = r < w && c < w;
-        const float v = in ? LU[static_cast<int64_t>(sb + c) * lda + sb + r] : 0.0f;   // LU(r, c)
+        const float lv = LU[static_cast<int64_t>(sb + min(c, w - 1)) * lda + sb + min(r, w - 1)];
+        const float v = in ? lv : 0.0f;   // LU(r, c)
         ML[r * 33 + c] = r > c ? v : (r == c ? 1.0f : 0.0f);                // unit lower L
         MU[c * 33 + r] = r < c ? v : (r == c ? (in ? v : 1.0f) : 0.0f);     // U^T (lower)
     }
@@ -56,15 +59,35 @@ __global__ __launch_bounds__(256) void trsm_u_strip_kernel(int b, int64_t n, con
     const int tid = threadIdx.x;
     const int64_t j0 = static_cast<int64_t>(blockIdx.x) * kStrip;
     const int nc = n - j0 < kStrip ? static_cast<int>(n - j0) : kStrip;
-    for (int idx = tid; idx < b * kStrip; idx += 256) {
-        const int c = idx / b, r = idx % b;
-        X[r * kXS + c] = c < nc ? U[(j0 + c) * ldu + r] : 0.0f;
+    // load the strip through registers, 16 elements per thread per batch, with
+    // clamped (always valid) addresses so the loads issue together
+    const int tot = b * kStrip;
+    for (int base = 0; base < tot; base += 256 * 16) {
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = min(base + u * 256 + tid, tot - 1);
+            const int c = idx / b, r = idx % b;
+            v[u] = U[(j0 + min(c, nc - 1)) * ldu + r];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = base + u * 256 + tid;
+            const int c = idx / b, r = idx % b;
+            if (idx < tot) X[r * kXS + c] = c < nc ? v[u] : 0.0f;
+        }
     }
     const int cc = tid & 31, g = tid >> 5;   // sub-block product: column cc, rows 4g..4g+3
     for (int kb = 0; kb < b; kb += kStrip) {
         const int w = b - kb < kStrip ? b - kb : kStrip;
         const float *src = inv + static_cast<int64_t>(kb / kStrip) * kInvBlock;
-        for (int e = tid; e < 32 * 32; e += 256) Li[(e / 32) * 33 + e % 32] = src[e];
+        {
+            float v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = src[u * 256 + tid];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) Li[((u * 256 + tid) / 32) * 33 + (u * 256 + tid) % 32] = v[u];
+        }
         __syncthreads();
         float out[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int r = 0; r < w; ++r) {
@@ -90,7 +113,8 @@ __global__ __launch_bounds__(256) void trsm_u_strip_kernel(int b, int64_t n, con
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     const int i = r0 + rs + 64 * q;
-                    const float l = i < b ? lcol[i] : 0.0f;
+                    const float lv = lcol[min(i, b - 1)];
+                    const float l = i < b ? lv : 0.0f;
 #pragma unroll
                     for (int c = 0; c < 8; ++c) acc[q][c] = fmaf(l, xv[c], acc[q][c]);
                 }
@@ -129,15 +153,33 @@ __global__ __launch_bounds__(256) void trsm_l_strip_kernel(int64_t m, int b, con
     const int tid = threadIdx.x;
     const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kStrip;
     const int nr = m - i0 < kStrip ? static_cast<int>(m - i0) : kStrip;
-    for (int idx = tid; idx < b * kStrip; idx += 256) {
-        const int c = idx / kStrip, r = idx % kStrip;
-        X[c * kXS + r] = r < nr ? L[static_cast<int64_t>(c) * ldl + i0 + r] : 0.0f;
+    const int tot = b * kStrip;
+    for (int base = 0; base < tot; base += 256 * 16) {
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = min(base + u * 256 + tid, tot - 1);
+            const int c = idx / kStrip, r = idx % kStrip;
+            v[u] = L[static_cast<int64_t>(c) * ldl + i0 + min(r, nr - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = base + u * 256 + tid;
+            const int c = idx / kStrip, r = idx % kStrip;
+            if (idx < tot) X[c * kXS + r] = r < nr ? v[u] : 0.0f;
+        }
     }
     const int rr = tid & 31, g = tid >> 5;   // sub-block product: row rr, columns 4g..4g+3
     for (int jb = 0; jb < b; jb += kStrip) {
         const int w = b - jb < kStrip ? b - jb : kStrip;
         const float *src = inv + static_cast<int64_t>(jb / kStrip) * kInvBlock + 1024;
-        for (int e = tid; e < 32 * 32; e += 256) Ui[(e / 32) * 33 + e % 32] = src[e];
+        {
+            float v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = src[u * 256 + tid];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) Ui[((u * 256 + tid) / 32) * 33 + (u * 256 + tid) % 32] = v[u];
+        }
         __syncthreads();
         float out[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int l = 0; l < w; ++l) {
@@ -155,7 +197,10 @@ __global__ __launch_bounds__(256) void trsm_l_strip_kernel(int64_t m, int b, con
             float u[kStrip];
             const float *ucol = Uk + static_cast<int64_t>(c) * ldu + jb;
 #pragma unroll
-            for (int l = 0; l < kStrip; ++l) u[l] = l < w ? ucol[l] : 0.0f;
+            for (int l = 0; l < kStrip; ++l) {
+                const float v = ucol[min(l, w - 1)];
+                u[l] = l < w ? v : 0.0f;
+            }
             for (int r = 0; r < kStrip; ++r) {
                 float acc = 0.0f;
 #pragma unroll

@@ -196,7 +196,10 @@ def test_quantize_bit_exact(ctx, oracle, fmt, side):
 # ------------------------------------------------------------------ Schur GEMM
 @pytest.mark.parametrize("fmt,m,n,nsl,tb", [("f16", 300, 260, 1, 128), ("f16", 1024, 896, 2, 128),
                                             ("f16t", 1024, 768, 3, 256), ("e4m3", 768, 1024, 2, 256),
-                                            ("e4m3", 1000, 520, 1, 128)])
+                                            ("e4m3", 1000, 520, 1, 128),
+                                            # >= 512 tiles of 256 x 256: the streamed large-tile kernel,
+                                            # with ragged edges in both dimensions
+                                            ("f16", 5990, 5900, 2, 128), ("e4m3", 6000, 5950, 2, 256)])
 def test_schur_update_vs_fp64(ctx, oracle, fmt, m, n, nsl, tb):
     import hplai
     import torch
