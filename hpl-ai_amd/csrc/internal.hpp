@@ -66,6 +66,8 @@ int launch_schur(int fmt, int64_t m, int64_t n, int nslices, int kslice, const v
                  int64_t ldl, const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
                  const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, int num_cu,
                  hipStream_t s);
+// true when launch_schur runs the streamed 256 x 256 kernel for this shape
+bool schur_large_tiles(int fmt, int64_t m, int64_t n, int tb, int num_cu);
 
 int launch_otf_gemv(int64_t n, uint64_t seed, const double *d, double alpha, const double *x,
                     double *y, double *work, hipStream_t s);

@@ -13,12 +13,17 @@
 // E4M3 : v_mfma_scale_f32_32x32x64_f8f6f4 with the tile exponents as E8M0 block
 //        scales, so the products come out unscaled.
 //
-// Workgroup: 256 threads (4 waves, 2 x 2), tile BM x BN (256 x 256 for the
-// large updates, 128 x 128 otherwise); each wave computes (BM/2) x (BN/2) as
-// (BM/64) x (BN/64) MFMA blocks of 32 x 32.  128 bytes of K per row per stage
-// for both formats (64 FP16 / 128 FP8), double-buffered in LDS with a 16-byte
-// row pad.  A 256 x 256 tile reads 8 KB of LDS per 16 MFMAs per wave, which
-// keeps LDS traffic below the MFMA time.
+// Two kernels, both 256 threads (4 waves, 2 x 2) with each wave computing a
+// quarter of the tile as 32 x 32 MFMA blocks:
+//  - schur_dma_kernel: 256 x 256 tiles for the large trailing updates;
+//    operands are streamed global -> LDS by direct loads, four LDS stages;
+//  - schur_kernel: 128 x 128 tiles (register-staged, double-buffered LDS) for
+//    thin updates and for the per-tile FP16 form.
+// Either way the kernel's HBM traffic per launch is (M + N) K bytes of
+// operands plus 8 M N bytes of C (read and written once).
+
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -339,74 +344,137 @@ __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
 
     const int row_tile = tb > 0 ? static_cast<int>(m0 / tb) : 0;
     const int col_tile = tb > 0 ? static_cast<int>(n0 / tb) : 0;
-    int scale_a = 127, scale_b = 127;
+
+    // operand registers of one stage; two sets alternate so the reads of
+    // stage ks + 1 overlap the second half of stage ks's MFMAs
+    struct Ops {
+        typename std::conditional<FMT == HPLAI_FMT_E4M3, intx8_t, half8_t>::type a[2][4], b[2][4];
+        int sa, sb;   // E4M3 block scales of the stage
+    };
+    constexpr int KK = FMT == HPLAI_FMT_E4M3 ? 1 : 2;   // MFMA K steps per stage
 
     // operand read offsets inside a stage (swizzled chunk positions)
     const int f = (lrow >> 2) & 3;
     const int a_base = (wm * (BM / 2) + lrow) * SB;
     const int b_base = BM * SB + (wn * (BN / 2) + lrow) * SB;
 
-    // prologue: stages 0..2 (clamped; extra copies land in buffers nobody reads)
-#pragma unroll
-    for (int st = 0; st < kDmaStages - 1; ++st) issue(st < nstages ? st : nstages - 1, st);
-
-    for (int ks = 0; ks < nstages; ++ks) {
-        const int buf = ks & (kDmaStages - 1);
-        // stage ks has landed once at most the two later stages' loads remain;
-        // the barrier publishes every wave's part and retires all reads of the
-        // buffer the next load overwrites
-        asm volatile("s_waitcnt vmcnt(16)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-        {
-            const int nxt = ks + kDmaStages - 1;
-            issue(nxt < nstages ? nxt : nstages - 1, nxt & (kDmaStages - 1));
-        }
-        if (FMT == HPLAI_FMT_E4M3 && ks % stages_per_slice == 0) {
-            const int sl = ks / stages_per_slice;
-            scale_a = 127 - eL[sl * ldel + row_tile];
-            scale_b = 127 - eU[sl * ldeu + col_tile];
-        }
+    // slice bookkeeping of the stage being read next (E4M3 scales change per slice)
+    int rd_slice = 0, rd_in_slice = 0;
+    auto read = [&](int buf, Ops &o) {
         const uint8_t *St = S + buf * kDmaStage;
-        if (FMT == HPLAI_FMT_E4M3) {
+        if constexpr (FMT == HPLAI_FMT_E4M3) {
             const int off_lo = 16 * ((2 * lgrp) ^ f), off_hi = 16 * ((2 * lgrp + 1) ^ f);
-            intx8_t a[4], b[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const uint8_t *p = St + a_base + i * 32 * SB;
                 const intx4_t lo = *reinterpret_cast<const intx4_t *>(p + off_lo);
                 const intx4_t hi = *reinterpret_cast<const intx4_t *>(p + off_hi);
-                a[i] = intx8_t{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+                o.a[0][i] = intx8_t{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const uint8_t *p = St + b_base + j * 32 * SB;
                 const intx4_t lo = *reinterpret_cast<const intx4_t *>(p + off_lo);
                 const intx4_t hi = *reinterpret_cast<const intx4_t *>(p + off_hi);
-                b[j] = intx8_t{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+                o.b[0][j] = intx8_t{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
             }
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                        a[i], b[j], acc[i][j], 0, 0, 0, scale_a, 0, scale_b);
         } else {
 #pragma unroll
             for (int kk = 0; kk < 2; ++kk) {
                 const int off = 16 * ((2 * kk + lgrp) ^ f);
-                half8_t a[4], b[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    a[i] = *reinterpret_cast<const half8_t *>(St + a_base + i * 32 * SB + off);
+                    o.a[kk][i] = *reinterpret_cast<const half8_t *>(St + a_base + i * 32 * SB + off);
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    b[j] = *reinterpret_cast<const half8_t *>(St + b_base + j * 32 * SB + off);
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+                    o.b[kk][j] = *reinterpret_cast<const half8_t *>(St + b_base + j * 32 * SB + off);
             }
         }
+    };
+    // E4M3 scales: the exponents of the stage after next are fetched one step
+    // early and consumed right after the workgroup barrier, where no LDS read
+    // is outstanding (a scalar-load wait would otherwise also wait for them)
+    int pe_l = 0, pe_u = 0;
+    auto fetch_exps = [&]() {
+        if constexpr (FMT == HPLAI_FMT_E4M3) {
+            // fetches past the last stage (never used) keep to the last slice
+            const int sl = rd_slice < nslices ? rd_slice : nslices - 1;
+            pe_l = eL[sl * ldel + row_tile];
+            pe_u = eU[sl * ldeu + col_tile];
+        }
+        if (++rd_in_slice == stages_per_slice) {
+            rd_in_slice = 0;
+            ++rd_slice;
+        }
+    };
+    auto take_exps = [&](Ops &o) {
+        o.sa = 127 - pe_l;
+        o.sb = 127 - pe_u;
+    };
+    // MFMAs of accumulator rows i0, i0 + 1
+    auto mma = [&](const Ops &o, int i0) {
+#pragma unroll
+        for (int kk = 0; kk < KK; ++kk)
+#pragma unroll
+            for (int i = i0; i < i0 + 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if constexpr (FMT == HPLAI_FMT_E4M3)
+                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                            o.a[kk][i], o.b[kk][j], acc[i][j], 0, 0, 0, o.sa, 0, o.sb);
+                    else
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(o.a[kk][i], o.b[kk][j],
+                                                                           acc[i][j], 0, 0, 0);
+                }
+    };
+    // One stage: first half of the MFMAs, then (once stage ks + 1 has landed
+    // and every wave has its stage-ks operands in registers) refill the
+    // stage-ks buffer with stage ks + 4 and read stage ks + 1 while the second
+    // half runs.  At that wait the loads of stages ks + 2 and ks + 3 (8 each
+    // per thread) may still be in flight.
+    // (The last stage also reads the clamped copy in the next buffer; the
+    // result is never used.)  The scheduling barriers keep the MFMAs, which
+    // touch no memory, on their side of the workgroup barrier.
+    auto step = [&](int ks, const Ops &cur, Ops &nxt) {
+        mma(cur, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("s_waitcnt vmcnt(16)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        take_exps(nxt);
+        fetch_exps();
+        __builtin_amdgcn_sched_barrier(0);
+        read((ks + 1) & (kDmaStages - 1), nxt);
+        const int ld = ks + kDmaStages;
+        issue(ld < nstages ? ld : nstages - 1, ld & (kDmaStages - 1));
+        mma(cur, 2);
+        // operand reads first, spread between the MFMAs, then the refill loads
+        constexpr int NM = 8 * KK;   // MFMAs in this half
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);   // 4 LDS reads
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // 1 MFMA
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            __builtin_amdgcn_sched_group_barrier(0x010, 2, 0);   // 2 refill loads
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // 1 MFMA
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, NM - 8, 0);
+        __builtin_amdgcn_sched_barrier(0);
+    };
+
+    // prologue: stages 0..3 (clamped; extra copies land in buffers nobody reads)
+#pragma unroll
+    for (int st = 0; st < kDmaStages; ++st) issue(st < nstages ? st : nstages - 1, st);
+    Ops r0, r1;
+    fetch_exps();               // stage 0
+    take_exps(r0);
+    fetch_exps();               // stage 1
+    asm volatile("s_waitcnt vmcnt(24)\n\ts_barrier" ::: "memory");
+    read(0, r0);
+    // nstages is even (K per launch is a multiple of 128 bytes)
+    for (int ks = 0; ks < nstages; ks += 2) {
+        step(ks, r0, r1);
+        step(ks + 1, r1, r0);
     }
     // drain the clamped tail loads before the workgroup can exit
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -495,11 +563,19 @@ static int schur_launch(int64_t m, int64_t n, int nslices, int kslice, const voi
     return HPLAI_OK;
 }
 
-// 256 x 256 tiles for updates that fill the device; 128 x 128 for thin ones.
-// With per-tile exponents every workgroup tile must lie inside one tb x tb
-// tile, so the large tiles also need tb % 256 == 0.  The per-tile FP16 form
-// rescales its accumulators between slices and keeps the 128 x 128 tile
-// (the 256 x 256 accumulators live in AGPRs and would have to be copied out).
+// The streamed 256 x 256 kernel for updates with at least two tiles per CU;
+// the 128 x 128 kernel for thin ones.  With per-tile exponents every
+// workgroup tile must lie inside one tb x tb tile, so the large tiles also
+// need tb % 256 == 0.  The per-tile FP16 form rescales its accumulators
+// between slices and keeps the 128 x 128 tile.
+bool schur_large_tiles(int fmt, int64_t m, int64_t n, int tb, int num_cu)
+{
+    if (fmt == HPLAI_FMT_F16_TILE) return false;
+    if (fmt == HPLAI_FMT_E4M3 && tb % kDmaTile != 0) return false;
+    if (num_cu <= 0) num_cu = 256;
+    return ceil_div(m, kDmaTile) * ceil_div(n, kDmaTile) >= 2 * num_cu;
+}
+
 template <int FMT>
 static int schur_dispatch(int64_t m, int64_t n, int nslices, int kslice, const void *Lq, int64_t ldl,
                           const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
@@ -507,8 +583,7 @@ static int schur_dispatch(int64_t m, int64_t n, int nslices, int kslice, const v
                           hipStream_t s)
 {
     if constexpr (FMT != HPLAI_FMT_F16_TILE) {
-        const bool tiles_ok = (FMT == HPLAI_FMT_F16) || (tb % kDmaTile == 0);
-        if (tiles_ok && ceil_div(m, kDmaTile) * ceil_div(n, kDmaTile) >= 2 * num_cu)
+        if (schur_large_tiles(FMT, m, n, tb, num_cu))
             return schur_dma_launch<FMT>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu,
                                          tb, C, ldc, s);
     }

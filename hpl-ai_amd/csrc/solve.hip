@@ -80,9 +80,10 @@ double now_s()
 struct GemmTimer {
     bool on = false;
     std::vector<hipEvent_t> ev;   // start/stop pairs
-    std::vector<double> flops;
+    std::vector<double> flops, bytes;
+    std::vector<char> large;
     int rc = HPLAI_OK;
-    void begin(hipStream_t s, double f)
+    void begin(hipStream_t s, double f, double by, bool is_large)
     {
         if (!on || rc) return;
         hipEvent_t a = nullptr, b = nullptr;
@@ -93,6 +94,8 @@ struct GemmTimer {
         ev.push_back(a);
         ev.push_back(b);
         flops.push_back(f);
+        bytes.push_back(by);
+        large.push_back(is_large ? 1 : 0);
         (void)hipEventRecord(a, s);
     }
     void end(hipStream_t s)
@@ -103,15 +106,27 @@ struct GemmTimer {
     // after synchronisation
     void collect(hplai_result *r)
     {
-        double ms = 0.0, fl = 0.0;
+        double ms = 0.0, fl = 0.0, lms = 0.0, lfl = 0.0, lby = 0.0;
+        int64_t ln = 0;
         for (size_t i = 0; i + 1 < ev.size(); i += 2) {
             float t = 0.0f;
-            if (hipEventElapsedTime(&t, ev[i], ev[i + 1]) == hipSuccess) ms += t;
+            if (hipEventElapsedTime(&t, ev[i], ev[i + 1]) != hipSuccess) t = 0.0f;
+            ms += t;
             fl += flops[i / 2];
+            if (large[i / 2]) {
+                ++ln;
+                lms += t;
+                lfl += flops[i / 2];
+                lby += bytes[i / 2];
+            }
         }
         r->schur_launches = static_cast<int64_t>(flops.size());
         r->schur_flops = fl;
         r->schur_ms = ms;
+        r->schur_large_launches = ln;
+        r->schur_large_flops = lfl;
+        r->schur_large_ms = lms;
+        r->schur_large_bytes = lby;
     }
     ~GemmTimer()
     {
@@ -163,7 +178,10 @@ struct LU {
              hipStream_t s) const
     {
         if (m <= 0 || nn <= 0) return HPLAI_OK;
-        timer->begin(s, 2.0 * static_cast<double>(m) * static_cast<double>(nn) * nsl * b);
+        const double K = static_cast<double>(nsl) * b;
+        timer->begin(s, 2.0 * static_cast<double>(m) * static_cast<double>(nn) * K,
+                     static_cast<double>(m + nn) * K * es + 8.0 * static_cast<double>(m) * nn,
+                     schur_large_tiles(fmt, m, nn, b, num_cu));
         const int rc = launch_schur(fmt, m, nn, nsl, kpad, lq(buf, row0, t0), ldq, el(buf, t0, row0),
                                     ntiles, uq(buf, col0, t0), ldq, eu(buf, t0, col0), ntiles, b,
                                     a(row0, col0), lda, num_cu, s);

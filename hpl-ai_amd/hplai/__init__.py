@@ -37,7 +37,9 @@ class Result(ctypes.Structure):
     _fields_ = [("time_gen", _dbl), ("time_lu", _dbl), ("time_ir", _dbl), ("time_total", _dbl),
                 ("gflops", _dbl), ("ir_steps", _int), ("residual", _dbl), ("harness", _dbl),
                 ("trace_resid", _dbl * 64), ("trace_harness", _dbl * 64), ("checksum", _dbl * 3),
-                ("schur_launches", _i64), ("schur_flops", _dbl), ("schur_ms", _dbl)]
+                ("schur_launches", _i64), ("schur_flops", _dbl), ("schur_ms", _dbl),
+                ("schur_large_launches", _i64), ("schur_large_flops", _dbl), ("schur_large_ms", _dbl),
+                ("schur_large_bytes", _dbl)]
 
 
 # (name, restype, argtypes) for every exported entry point of include/hplai.h
@@ -208,7 +210,9 @@ class Context:
                     time_total=r.time_total, gflops=r.gflops, ir_steps=k, residual=r.residual,
                     harness=r.harness, trace_resid=list(r.trace_resid[:k]),
                     trace_harness=list(r.trace_harness[:k]), checksum=list(r.checksum),
-                    schur_launches=r.schur_launches, schur_flops=r.schur_flops, schur_ms=r.schur_ms)
+                    schur_launches=r.schur_launches, schur_flops=r.schur_flops, schur_ms=r.schur_ms,
+                    schur_large_launches=r.schur_large_launches, schur_large_flops=r.schur_large_flops,
+                    schur_large_ms=r.schur_large_ms, schur_large_bytes=r.schur_large_bytes)
 
     def solution_x(self, n):
         import numpy as np

@@ -156,6 +156,12 @@ typedef struct hplai_result {
     int64_t schur_launches;
     double  schur_flops;
     double  schur_ms;
+    /* the subset of those launches that ran the streamed 256 x 256 kernel, and
+       their algorithmic HBM bytes ((m + n) K operand bytes + 8 m n for C) */
+    int64_t schur_large_launches;
+    double  schur_large_flops;
+    double  schur_large_ms;
+    double  schur_large_bytes;
 } hplai_result;
 
 /* Generate, factor and refine; synchronous.  Returns HPLAI_ENOCONV if the

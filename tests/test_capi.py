@@ -52,8 +52,9 @@ def _c_layout(tmp_path):
 #include <stddef.h>
 #include "hplai.h"
 int main(void) {
-  printf("%zu %zu %zu %zu\n", sizeof(hplai_params), offsetof(hplai_params, profile),
-         sizeof(hplai_result), offsetof(hplai_result, schur_ms));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(hplai_params), offsetof(hplai_params, profile),
+         sizeof(hplai_result), offsetof(hplai_result, schur_ms),
+         offsetof(hplai_result, schur_large_bytes));
   return 0;
 }''')
     exe = tmp_path / "layout"
@@ -64,11 +65,12 @@ int main(void) {
 
 def test_struct_layouts_match_c(tmp_path):
     import hplai
-    sp, op, sr, orr = _c_layout(tmp_path)
+    sp, op, sr, orr, orl = _c_layout(tmp_path)
     assert ctypes.sizeof(hplai.Params) == sp
     assert hplai.Params.profile.offset == op
     assert ctypes.sizeof(hplai.Result) == sr
     assert hplai.Result.schur_ms.offset == orr
+    assert hplai.Result.schur_large_bytes.offset == orl
 
 
 def test_driver_help():
