@@ -120,9 +120,12 @@ def main():
     ap.add_argument("--compare", default="f16", help="second form run beside the primary ('' = none)")
     ap.add_argument("--compare-outer", type=int, default=8)
     ap.add_argument("--no-lookahead", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=8192)
+    ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--cpu-b", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-check-bytes", type=int, default=0,
+                    help="first run one device-to-device copy of this many bytes (calibrates "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE against a known traffic)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -134,6 +137,13 @@ def main():
     import hplai
 
     torch.cuda.set_device(0)
+    if args.copy_check_bytes > 0:
+        src = torch.ones(args.copy_check_bytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 1.0, out=dst)   # an elementwise kernel: reads and writes the bytes once
+        torch.cuda.synchronize()
+        del src, dst
     ctx = hplai.Context(0)
     # load the code objects once (n = 2048) so the first timed solve does not pay for it
     ctx.solve(2048, 512 if args.b >= 512 else args.b, fmt=args.fmt, outer=1)

@@ -1,0 +1,112 @@
+"""Summaries of rocprofv3 output for the solver's kernels.
+
+    python -m hplai.profsum stats  KERNEL_STATS_CSV [--top N]
+    python -m hplai.profsum traffic --fetch FETCH_CSV --write WRITE_CSV
+                                    --kernel NAME_SUBSTRING --copy-bytes BYTES
+
+`stats` reads the `*_kernel_stats.csv` written by `rocprofv3 --kernel-trace
+--stats --output-format csv` and prints one line per kernel (calls, total and
+average duration, share of GPU time).
+
+`traffic` reads two `*_counter_collection.csv` files written by
+`rocprofv3 --pmc FETCH_SIZE` and `rocprofv3 --pmc WRITE_SIZE` over the same
+command.  The command must also run one plain device-to-device copy of
+BYTES bytes (bench.py --copy-check-bytes) so that the counters can be
+calibrated: the factor is what FETCH_SIZE (WRITE_SIZE) reads for that copy
+divided by BYTES.  The HBM bytes per launch of the kernel whose name contains
+NAME_SUBSTRING are then FETCH/factor_f + WRITE/factor_w, averaged over its
+launches.
+"""
+import argparse
+import csv
+import sys
+from collections import defaultdict
+
+
+def _rows(path):
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def stats(path, top):
+    rows = _rows(path)
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    print("%-70s %8s %12s %12s %7s" % ("kernel", "calls", "total_ms", "avg_us", "share"))
+    for r in rows[:top]:
+        name = r["Name"]
+        if len(name) > 70:
+            name = name[:67] + "..."
+        print("%-70s %8d %12.3f %12.3f %6.2f%%" % (
+            name, int(r["Calls"]), float(r["TotalDurationNs"]) * 1e-6,
+            float(r["AverageNs"]) * 1e-3, 100.0 * float(r["TotalDurationNs"]) / total))
+
+
+def _per_dispatch(path, counter):
+    """{dispatch_id: (kernel_name, value in bytes)} for one counter (KiB in the file)."""
+    out = {}
+    acc = defaultdict(float)
+    names = {}
+    for r in _rows(path):
+        if r.get("Counter_Name") != counter:
+            continue
+        d = int(r["Dispatch_Id"])
+        acc[d] += float(r["Counter_Value"])
+        names[d] = r["Kernel_Name"]
+    for d, v in acc.items():
+        out[d] = (names[d], v * 1024.0)
+    return out
+
+
+def _copy_factor(per, copy_bytes):
+    # the calibration copy is the dispatch whose counter is closest to copy_bytes
+    # among the elementwise copy kernels
+    cands = [(abs(v - copy_bytes), v) for name, v in per.values()
+             if "copy" in name.lower() or "elementwise" in name.lower()]
+    if not cands:
+        raise SystemExit("no copy kernel found for calibration")
+    return min(cands)[1] / copy_bytes
+
+
+def traffic(fetch, write, kernel, copy_bytes):
+    pf = _per_dispatch(fetch, "FETCH_SIZE")
+    pw = _per_dispatch(write, "WRITE_SIZE")
+    ff = _copy_factor(pf, copy_bytes)
+    fw = _copy_factor(pw, copy_bytes)
+    kf = [v for name, v in pf.values() if kernel in name]
+    kw = [v for name, v in pw.values() if kernel in name]
+    if not kf or not kw:
+        raise SystemExit("kernel %r not found" % kernel)
+    fetch_per = sum(kf) / len(kf) / ff
+    write_per = sum(kw) / len(kw) / fw
+    print("kernel %s" % kernel)
+    print("launches_fetch_run %d" % len(kf))
+    print("launches_write_run %d" % len(kw))
+    print("copy_calibration_bytes %d" % copy_bytes)
+    print("fetch_factor %.4f" % ff)
+    print("write_factor %.4f" % fw)
+    print("fetch_bytes_per_launch %.6g" % fetch_per)
+    print("write_bytes_per_launch %.6g" % write_per)
+    print("hbm_bytes_per_launch %.6g" % (fetch_per + write_per))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m hplai.profsum")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("csv")
+    s.add_argument("--top", type=int, default=25)
+    t = sub.add_parser("traffic")
+    t.add_argument("--fetch", required=True)
+    t.add_argument("--write", required=True)
+    t.add_argument("--kernel", required=True)
+    t.add_argument("--copy-bytes", type=int, required=True)
+    a = ap.parse_args(argv)
+    if a.cmd == "stats":
+        stats(a.csv, a.top)
+    else:
+        traffic(a.fetch, a.write, a.kernel, a.copy_bytes)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
