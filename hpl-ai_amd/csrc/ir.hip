@@ -93,106 +93,150 @@ int launch_otf_gemv(int64_t n, uint64_t seed, const double *d, double alpha, con
 // ---------------------------------------------------------------------------
 // Triangular solves (panel_trsvL / panel_trsvU, panel_trsv.hpp:310-758, with
 // tttrsvL/U, 115-153, and ttgemv, 10-45): FP32 factor entries widened to FP64.
-// Per block of bs rows: (1) one workgroup solves the diagonal block with x in
-// LDS, 32-row sub-blocks solved by one wave with shuffles; (2) a GEMV updates
-// the rest of the vector with the block's column panel.
+// Per block of bs rows: (1) one workgroup solves the diagonal block, (2) a
+// GEMV updates the rest of the vector with the block's column panel.
+//
+// Diagonal block: thread t owns rows t + 256 k and keeps their x in
+// registers.  The block is walked in 32-column strips; every thread holds its
+// rows' 32 entries of the current strip in registers and prefetches the next
+// strip while the current one is used.  The 32 x 32 diagonal sub-block is
+// solved by the half-wave that owns its rows: x_c is broadcast with
+// v_readlane (wave-uniform lane index) and the lanes below subtract
+// l_ic * x_c — 32 dependent steps of a few cycles each.  The strip's final x
+// goes through LDS to the threads owning the rows further on, which subtract
+// their 32-term products.  The upper solve multiplies by 1 / u_cc computed
+// before the dependent chain (one rounding different from a division).
 // ---------------------------------------------------------------------------
 constexpr int kTrsvMax = 1024;
 
-__global__ __launch_bounds__(256) void trsv_diag_lower_kernel(int bs, const float *__restrict__ Lkk,
-                                                              int64_t lda, double *__restrict__ x)
+__device__ __forceinline__ double readlane_f64(double v, int lane)
 {
-    __shared__ double xs[kTrsvMax];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int r = tid; r < bs; r += 256) xs[r] = x[r];
-    __syncthreads();
-    for (int sb = 0; sb < bs; sb += 32) {
-        const int w = bs - sb < 32 ? bs - sb : 32;
-        if (wave == 0) {
-            // lane i holds x_i and row i of the 32 x 32 sub-block (loaded up front)
-            double xr = lane < w ? xs[sb + lane] : 0.0;
-            float lrow[32];
-#pragma unroll
-            for (int c = 0; c < 32; ++c)
-            {
-                const float v = Lkk[static_cast<int64_t>(sb + min(c, w - 1)) * lda + sb + min(lane, w - 1)];
-                lrow[c] = (lane < w && c < w) ? v : 0.0f;
-            }
-#pragma unroll
-            for (int c = 0; c < 32; ++c) {
-                if (c < w) {
-                    const double xc = __shfl(xr, c);
-                    if (lane > c && lane < w) xr = xr - xc * static_cast<double>(lrow[c]);
-                }
-            }
-            if (lane < w) xs[sb + lane] = xr;
-        }
-        __syncthreads();
-        for (int r = sb + w + tid; r < bs; r += 256) {
-            float lv[32];   // all 32 loads in flight at once
-#pragma unroll
-            for (int c = 0; c < 32; ++c) {
-                const float v = Lkk[static_cast<int64_t>(sb + min(c, w - 1)) * lda + r];
-                lv[c] = c < w ? v : 0.0f;
-            }
-            double acc = 0.0;
-#pragma unroll
-            for (int c = 0; c < 32; ++c)
-                if (c < w) acc = fma(static_cast<double>(lv[c]), xs[sb + c], acc);
-            xs[r] -= acc;
-        }
-        __syncthreads();
-    }
-    for (int r = tid; r < bs; r += 256) x[r] = xs[r];
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
 }
 
-__global__ __launch_bounds__(256) void trsv_diag_upper_kernel(int bs, const float *__restrict__ Ukk,
-                                                              int64_t lda, double *__restrict__ x)
+// strip loader: dst[k][c] = M[(col0 + c), row t + 256 k], clamped in-bounds
+template <int RPT>
+__device__ __forceinline__ void trsv_load_strip(const float *__restrict__ M, int64_t lda, int bs,
+                                                int col0, int tid, float (&dst)[RPT][32])
 {
-    __shared__ double xs[kTrsvMax];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int r = min(tid + 256 * k, bs - 1);
+#pragma unroll
+        for (int c = 0; c < 32; ++c) dst[k][c] = M[static_cast<int64_t>(min(col0 + c, bs - 1)) * lda + r];
+    }
+}
+
+template <int RPT, bool UPPER>
+__global__ __launch_bounds__(256) void trsv_diag_kernel(int bs, const float *__restrict__ T, int64_t lda,
+                                                        double *__restrict__ x)
+{
+    __shared__ double xs[32];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int r = tid; r < bs; r += 256) xs[r] = x[r];
-    __syncthreads();
+    double xr[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int r = tid + 256 * k;
+        xr[k] = r < bs ? x[r] : 0.0;
+    }
     const int nsb = (bs + 31) / 32;
-    for (int q = nsb - 1; q >= 0; --q) {
+    constexpr bool PF = RPT <= 2;   // register room to prefetch the next strip
+    float cur[RPT][32], nxt[PF ? RPT : 1][32];
+    trsv_load_strip<RPT>(T, lda, bs, (UPPER ? nsb - 1 : 0) * 32, tid, cur);
+    for (int step = 0; step < nsb; ++step) {
+        const int q = UPPER ? nsb - 1 - step : step;
         const int sb = q * 32;
         const int w = bs - sb < 32 ? bs - sb : 32;
-        if (wave == 0) {
-            double xr = lane < w ? xs[sb + lane] : 0.0;
-            float urow[32];
+        if constexpr (PF)
+            if (step + 1 < nsb) trsv_load_strip<RPT>(T, lda, bs, (UPPER ? q - 1 : q + 1) * 32, tid, nxt);
+        // ---- the owner half-wave solves the 32 x 32 diagonal sub-block ----
+        const int kq = sb / 256, owner = (sb % 256) / 64, half = ((sb % 256) / 32) & 1;
+        if (wave == owner) {
+            const int i = lane - 32 * half;                // row sb + i for the owning lanes
+            const bool mine = i >= 0 && i < w;
+            double v = 0.0, rinv = 1.0;
+            float rowv[32];
 #pragma unroll
-            for (int c = 0; c < 32; ++c)
-            {
-                const float v = Ukk[static_cast<int64_t>(sb + min(c, w - 1)) * lda + sb + min(lane, w - 1)];
-                urow[c] = (lane < w && c < w) ? v : 0.0f;
+            for (int k = 0; k < RPT; ++k)
+                if (k == kq) {
+                    v = xr[k];
+#pragma unroll
+                    for (int c = 0; c < 32; ++c) rowv[c] = cur[k][c];
+                }
+            if (UPPER) {
+                float dg = 1.0f;
+#pragma unroll
+                for (int c = 0; c < 32; ++c)
+                    if (c == i) dg = rowv[c];
+                rinv = 1.0 / static_cast<double>(dg);
             }
+            if (!UPPER) {
 #pragma unroll
-            for (int c = 31; c >= 0; --c) {
-                if (c < w) {
-                    if (lane == c) xr = xr / static_cast<double>(urow[c]);
-                    const double xc = __shfl(xr, c);
-                    if (lane < c) xr = xr - xc * static_cast<double>(urow[c]);
+                for (int c = 0; c < 32; ++c) {
+                    if (c < w) {
+                        const double xc = readlane_f64(v, 32 * half + c);
+                        if (mine && i > c) v = fma(-static_cast<double>(rowv[c]), xc, v);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int c = 31; c >= 0; --c) {
+                    if (c < w) {
+                        if (i == c) v *= rinv;
+                        const double xc = readlane_f64(v, 32 * half + c);
+                        if (mine && i < c) v = fma(-static_cast<double>(rowv[c]), xc, v);
+                    }
                 }
             }
-            if (lane < w) xs[sb + lane] = xr;
+            if (mine) xs[i] = v;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k)
+                if (k == kq && mine) xr[k] = v;
         }
         __syncthreads();
-        for (int r = tid; r < sb; r += 256) {
-            float uv[32];
+        // ---- rows further on subtract the strip's contribution ----
 #pragma unroll
-            for (int c = 0; c < 32; ++c) {
-                const float v = Ukk[static_cast<int64_t>(sb + min(c, w - 1)) * lda + r];
-                uv[c] = c < w ? v : 0.0f;
+        for (int k = 0; k < RPT; ++k) {
+            const int r = tid + 256 * k;
+            const bool further = UPPER ? (r < sb) : (r >= sb + w && r < bs);
+            if (further) {
+                double acc = 0.0;
+#pragma unroll
+                for (int c = 0; c < 32; ++c)
+                    if (c < w) acc = fma(static_cast<double>(cur[k][c]), xs[c], acc);
+                xr[k] -= acc;
             }
-            double acc = 0.0;
-#pragma unroll
-            for (int c = 0; c < 32; ++c)
-                if (c < w) acc = fma(static_cast<double>(uv[c]), xs[sb + c], acc);
-            xs[r] -= acc;
         }
         __syncthreads();
+        if (step + 1 < nsb) {
+            if constexpr (PF) {
+#pragma unroll
+                for (int k = 0; k < RPT; ++k)
+#pragma unroll
+                    for (int c = 0; c < 32; ++c) cur[k][c] = nxt[k][c];
+            } else {
+                trsv_load_strip<RPT>(T, lda, bs, (UPPER ? q - 1 : q + 1) * 32, tid, cur);
+            }
+        }
     }
-    for (int r = tid; r < bs; r += 256) x[r] = xs[r];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int r = tid + 256 * k;
+        if (r < bs) x[r] = xr[k];
+    }
+}
+
+template <bool UPPER>
+static void launch_trsv_diag(int w, const float *T, int64_t lda, double *x, hipStream_t s)
+{
+    if (w <= 256)
+        hipLaunchKernelGGL((trsv_diag_kernel<1, UPPER>), dim3(1), dim3(256), 0, s, w, T, lda, x);
+    else if (w <= 512)
+        hipLaunchKernelGGL((trsv_diag_kernel<2, UPPER>), dim3(1), dim3(256), 0, s, w, T, lda, x);
+    else
+        hipLaunchKernelGGL((trsv_diag_kernel<4, UPPER>), dim3(1), dim3(256), 0, s, w, T, lda, x);
 }
 
 // y(0:m) -= P(m x w) * xb(0:w), P column-major (ttgemv, panel_trsv.hpp:10-45).
@@ -237,7 +281,7 @@ int launch_trsv_lower(int64_t n, int bs, const float *LU, int64_t lda, double *x
     for (int64_t kb = 0; kb < n; kb += bs) {
         const int w = n - kb < bs ? static_cast<int>(n - kb) : bs;
         const float *Lkk = LU + kb * lda + kb;
-        hipLaunchKernelGGL(trsv_diag_lower_kernel, dim3(1), dim3(256), 0, s, w, Lkk, lda, x + kb);
+        launch_trsv_diag<false>(w, Lkk, lda, x + kb, s);
         HPLAI_HIP(hipGetLastError());
         const int64_t m = n - kb - w;
         if (m > 0) {
@@ -256,7 +300,7 @@ int launch_trsv_upper(int64_t n, int bs, const float *LU, int64_t lda, double *x
         const int64_t kb = q * bs;
         const int w = n - kb < bs ? static_cast<int>(n - kb) : bs;
         const float *Ukk = LU + kb * lda + kb;
-        hipLaunchKernelGGL(trsv_diag_upper_kernel, dim3(1), dim3(256), 0, s, w, Ukk, lda, x + kb);
+        launch_trsv_diag<true>(w, Ukk, lda, x + kb, s);
         HPLAI_HIP(hipGetLastError());
         if (kb > 0) {
             hipLaunchKernelGGL(trsv_gemv_kernel, dim3(static_cast<unsigned>(ceil_div(kb, kGemvRowsT))),

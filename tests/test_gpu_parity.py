@@ -267,7 +267,7 @@ def test_otf_gemv_vs_oracle(ctx, oracle, n):
     assert np.max(np.abs(gy.cpu().numpy() - ref)) / scale < 1e-13
 
 
-@pytest.mark.parametrize("n,bs", [(1200, 60), (2048, 512), (1000, 100)])
+@pytest.mark.parametrize("n,bs", [(1200, 60), (2048, 512), (1000, 100), (2600, 512), (3000, 1000)])
 def test_trsv_vs_oracle(ctx, oracle, n, bs):
     LU, d = oracle.lu(oracle.LU_PRECOMP, n, bs if n % bs == 0 else 100, SEED)
     rng = np.random.default_rng(5)
