@@ -72,8 +72,16 @@ bool schur_large_tiles(int fmt, int64_t m, int64_t n, int tb, int num_cu);
 int launch_otf_gemv(int64_t n, uint64_t seed, const double *d, double alpha, const double *x,
                     double *y, double *work, hipStream_t s);
 int64_t otf_gemv_work_doubles(int64_t n);
-int launch_trsv_lower(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s);
-int launch_trsv_upper(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s);
+// triangular solves with the FP32 factors of LU, FP64 vectors; nrhs columns
+// of x with stride ldx
+int launch_trsv_lower(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s,
+                      int nrhs = 1, int64_t ldx = 0);   // x := L^-1 x
+int launch_trsv_upper(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s,
+                      int nrhs = 1, int64_t ldx = 0);   // x := U^-1 x
+int launch_trsv_upper_t(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s,
+                        int nrhs = 1, int64_t ldx = 0);   // x := U^-T x
+int launch_trsv_lower_t(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s,
+                        int nrhs = 1, int64_t ldx = 0);   // x := L^-T x
 int launch_norm_inf(int64_t n, const double *x, double *out, hipStream_t s);
 int launch_vec_div(int64_t n, const double *num, const double *den, double *out, hipStream_t s);
 int launch_vec_add(int64_t n, const double *x, double *y, hipStream_t s);

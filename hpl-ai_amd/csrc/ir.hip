@@ -116,8 +116,10 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
     return __hiloint2double(hi, lo);
 }
 
-// strip loader: dst[k][c] = M[(col0 + c), row t + 256 k], clamped in-bounds
-template <int RPT>
+// strip loader: dst[k][c] = element (row t + 256 k, column col0 + c) of the
+// block, clamped in-bounds; element (r, c) is M[c*lda + r], or M[r*lda + c]
+// for the transposed factor (TRANS)
+template <int RPT, bool TRANS>
 __device__ __forceinline__ void trsv_load_strip(const float *__restrict__ M, int64_t lda, int bs,
                                                 int col0, int tid, float (&dst)[RPT][32])
 {
@@ -125,15 +127,21 @@ __device__ __forceinline__ void trsv_load_strip(const float *__restrict__ M, int
     for (int k = 0; k < RPT; ++k) {
         const int r = min(tid + 256 * k, bs - 1);
 #pragma unroll
-        for (int c = 0; c < 32; ++c) dst[k][c] = M[static_cast<int64_t>(min(col0 + c, bs - 1)) * lda + r];
+        for (int c = 0; c < 32; ++c) {
+            const int64_t col = min(col0 + c, bs - 1);
+            dst[k][c] = TRANS ? M[static_cast<int64_t>(r) * lda + col] : M[col * lda + r];
+        }
     }
 }
 
-template <int RPT, bool UPPER>
+// Solves T x = x in place for the bs x bs triangle T (lower or upper, unit
+// or not) at the top-left of M; right-hand side blockIdx.y at x + y * ldx.
+template <int RPT, bool UPPER, bool UNIT, bool TRANS>
 __global__ __launch_bounds__(256) void trsv_diag_kernel(int bs, const float *__restrict__ T, int64_t lda,
-                                                        double *__restrict__ x)
+                                                        double *__restrict__ x, int64_t ldx)
 {
     __shared__ double xs[32];
+    x += static_cast<int64_t>(blockIdx.y) * ldx;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double xr[RPT];
 #pragma unroll
@@ -144,13 +152,14 @@ __global__ __launch_bounds__(256) void trsv_diag_kernel(int bs, const float *__r
     const int nsb = (bs + 31) / 32;
     constexpr bool PF = RPT <= 2;   // register room to prefetch the next strip
     float cur[RPT][32], nxt[PF ? RPT : 1][32];
-    trsv_load_strip<RPT>(T, lda, bs, (UPPER ? nsb - 1 : 0) * 32, tid, cur);
+    trsv_load_strip<RPT, TRANS>(T, lda, bs, (UPPER ? nsb - 1 : 0) * 32, tid, cur);
     for (int step = 0; step < nsb; ++step) {
         const int q = UPPER ? nsb - 1 - step : step;
         const int sb = q * 32;
         const int w = bs - sb < 32 ? bs - sb : 32;
         if constexpr (PF)
-            if (step + 1 < nsb) trsv_load_strip<RPT>(T, lda, bs, (UPPER ? q - 1 : q + 1) * 32, tid, nxt);
+            if (step + 1 < nsb)
+                trsv_load_strip<RPT, TRANS>(T, lda, bs, (UPPER ? q - 1 : q + 1) * 32, tid, nxt);
         // ---- the owner half-wave solves the 32 x 32 diagonal sub-block ----
         const int kq = sb / 256, owner = (sb % 256) / 64, half = ((sb % 256) / 32) & 1;
         if (wave == owner) {
@@ -165,7 +174,7 @@ __global__ __launch_bounds__(256) void trsv_diag_kernel(int bs, const float *__r
 #pragma unroll
                     for (int c = 0; c < 32; ++c) rowv[c] = cur[k][c];
                 }
-            if (UPPER) {
+            if (!UNIT) {
                 float dg = 1.0f;
 #pragma unroll
                 for (int c = 0; c < 32; ++c)
@@ -176,6 +185,7 @@ __global__ __launch_bounds__(256) void trsv_diag_kernel(int bs, const float *__r
 #pragma unroll
                 for (int c = 0; c < 32; ++c) {
                     if (c < w) {
+                        if (!UNIT && i == c) v *= rinv;
                         const double xc = readlane_f64(v, 32 * half + c);
                         if (mine && i > c) v = fma(-static_cast<double>(rowv[c]), xc, v);
                     }
@@ -184,7 +194,7 @@ __global__ __launch_bounds__(256) void trsv_diag_kernel(int bs, const float *__r
 #pragma unroll
                 for (int c = 31; c >= 0; --c) {
                     if (c < w) {
-                        if (i == c) v *= rinv;
+                        if (!UNIT && i == c) v *= rinv;
                         const double xc = readlane_f64(v, 32 * half + c);
                         if (mine && i < c) v = fma(-static_cast<double>(rowv[c]), xc, v);
                     }
@@ -217,7 +227,7 @@ __global__ __launch_bounds__(256) void trsv_diag_kernel(int bs, const float *__r
 #pragma unroll
                     for (int c = 0; c < 32; ++c) cur[k][c] = nxt[k][c];
             } else {
-                trsv_load_strip<RPT>(T, lda, bs, (UPPER ? q - 1 : q + 1) * 32, tid, cur);
+                trsv_load_strip<RPT, TRANS>(T, lda, bs, (UPPER ? q - 1 : q + 1) * 32, tid, cur);
             }
         }
     }
@@ -228,28 +238,32 @@ __global__ __launch_bounds__(256) void trsv_diag_kernel(int bs, const float *__r
     }
 }
 
-template <bool UPPER>
-static void launch_trsv_diag(int w, const float *T, int64_t lda, double *x, hipStream_t s)
+template <bool UPPER, bool UNIT, bool TRANS>
+static void launch_trsv_diag(int w, const float *T, int64_t lda, double *x, int64_t ldx, int nrhs,
+                             hipStream_t s)
 {
+    const dim3 grid(1, static_cast<unsigned>(nrhs));
     if (w <= 256)
-        hipLaunchKernelGGL((trsv_diag_kernel<1, UPPER>), dim3(1), dim3(256), 0, s, w, T, lda, x);
+        hipLaunchKernelGGL((trsv_diag_kernel<1, UPPER, UNIT, TRANS>), grid, dim3(256), 0, s, w, T, lda, x, ldx);
     else if (w <= 512)
-        hipLaunchKernelGGL((trsv_diag_kernel<2, UPPER>), dim3(1), dim3(256), 0, s, w, T, lda, x);
+        hipLaunchKernelGGL((trsv_diag_kernel<2, UPPER, UNIT, TRANS>), grid, dim3(256), 0, s, w, T, lda, x, ldx);
     else
-        hipLaunchKernelGGL((trsv_diag_kernel<4, UPPER>), dim3(1), dim3(256), 0, s, w, T, lda, x);
+        hipLaunchKernelGGL((trsv_diag_kernel<4, UPPER, UNIT, TRANS>), grid, dim3(256), 0, s, w, T, lda, x, ldx);
 }
 
 // y(0:m) -= P(m x w) * xb(0:w), P column-major (ttgemv, panel_trsv.hpp:10-45).
 // Workgroup = 64 rows x 4 column groups; the 4 partial sums of a row are added
-// in a fixed order through LDS.
+// in a fixed order through LDS.  Right-hand side blockIdx.y at offset y * ldx.
 constexpr int kGemvRowsT = 64;
 
 __global__ __launch_bounds__(256) void trsv_gemv_kernel(int64_t m, int w, const float *__restrict__ P,
                                                         int64_t lda, const double *__restrict__ xb,
-                                                        double *__restrict__ y)
+                                                        double *__restrict__ y, int64_t ldx)
 {
     __shared__ double xs[kTrsvMax];
     __shared__ double part[4][kGemvRowsT];
+    xb += static_cast<int64_t>(blockIdx.y) * ldx;
+    y += static_cast<int64_t>(blockIdx.y) * ldx;
     for (int c = threadIdx.x; c < w; c += 256) xs[c] = xb[c];
     __syncthreads();
     const int lr = threadIdx.x % kGemvRowsT, g = threadIdx.x / kGemvRowsT;
@@ -276,35 +290,114 @@ __global__ __launch_bounds__(256) void trsv_gemv_kernel(int64_t m, int w, const 
     if (g == 0 && r < m) y[r] -= ((part[0][lr] + part[1][lr]) + part[2][lr]) + part[3][lr];
 }
 
-int launch_trsv_lower(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s)
+// Transposed panel: y(r) -= sum_c P[r*lda + c] * xb(c), r < m, c < w — each
+// row is contiguous; one wave per row at a time, lanes split the columns and
+// the partial sums are combined by a fixed shuffle tree.
+constexpr int kGemvTRows = 16;
+
+__global__ __launch_bounds__(256) void trsv_gemv_t_kernel(int64_t m, int w, const float *__restrict__ P,
+                                                          int64_t lda, const double *__restrict__ xb,
+                                                          double *__restrict__ y, int64_t ldx)
+{
+    __shared__ double xs[kTrsvMax];
+    xb += static_cast<int64_t>(blockIdx.y) * ldx;
+    y += static_cast<int64_t>(blockIdx.y) * ldx;
+    for (int c = threadIdx.x; c < w; c += 256) xs[c] = xb[c];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int rr = wave; rr < kGemvTRows; rr += 4) {
+        const int64_t r = static_cast<int64_t>(blockIdx.x) * kGemvTRows + rr;
+        if (r >= m) break;
+        const float *pr = P + r * lda;
+        double acc = 0.0;
+        for (int c = lane; c < w; c += 64) acc = fma(static_cast<double>(pr[c]), xs[c], acc);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        if (lane == 0) y[r] -= acc;
+    }
+}
+
+static void launch_gemv(bool trans, int64_t m, int w, const float *P, int64_t lda, const double *xb, double *y,
+                        int64_t ldx, int nrhs, hipStream_t s)
+{
+    if (trans)
+        hipLaunchKernelGGL(trsv_gemv_t_kernel, dim3(static_cast<unsigned>(ceil_div(m, kGemvTRows)), nrhs),
+                           dim3(256), 0, s, m, w, P, lda, xb, y, ldx);
+    else
+        hipLaunchKernelGGL(trsv_gemv_kernel, dim3(static_cast<unsigned>(ceil_div(m, kGemvRowsT)), nrhs),
+                           dim3(256), 0, s, m, w, P, lda, xb, y, ldx);
+}
+
+// x := L^-1 x (unit lower L of LU), nrhs columns of x with stride ldx
+int launch_trsv_lower(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s, int nrhs,
+                      int64_t ldx)
 {
     for (int64_t kb = 0; kb < n; kb += bs) {
         const int w = n - kb < bs ? static_cast<int>(n - kb) : bs;
         const float *Lkk = LU + kb * lda + kb;
-        launch_trsv_diag<false>(w, Lkk, lda, x + kb, s);
+        launch_trsv_diag<false, true, false>(w, Lkk, lda, x + kb, ldx, nrhs, s);
         HPLAI_HIP(hipGetLastError());
         const int64_t m = n - kb - w;
         if (m > 0) {
-            hipLaunchKernelGGL(trsv_gemv_kernel, dim3(static_cast<unsigned>(ceil_div(m, kGemvRowsT))),
-                               dim3(256), 0, s, m, w, Lkk + w, lda, x + kb, x + kb + w);
+            launch_gemv(false, m, w, Lkk + w, lda, x + kb, x + kb + w, ldx, nrhs, s);
             HPLAI_HIP(hipGetLastError());
         }
     }
     return HPLAI_OK;
 }
 
-int launch_trsv_upper(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s)
+// x := U^-1 x (upper U of LU)
+int launch_trsv_upper(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s, int nrhs,
+                      int64_t ldx)
 {
     const int64_t nblk = ceil_div(n, bs);
     for (int64_t q = nblk - 1; q >= 0; --q) {
         const int64_t kb = q * bs;
         const int w = n - kb < bs ? static_cast<int>(n - kb) : bs;
         const float *Ukk = LU + kb * lda + kb;
-        launch_trsv_diag<true>(w, Ukk, lda, x + kb, s);
+        launch_trsv_diag<true, false, false>(w, Ukk, lda, x + kb, ldx, nrhs, s);
         HPLAI_HIP(hipGetLastError());
         if (kb > 0) {
-            hipLaunchKernelGGL(trsv_gemv_kernel, dim3(static_cast<unsigned>(ceil_div(kb, kGemvRowsT))),
-                               dim3(256), 0, s, kb, w, LU + kb * lda, lda, x + kb, x);
+            launch_gemv(false, kb, w, LU + kb * lda, lda, x + kb, x, ldx, nrhs, s);
+            HPLAI_HIP(hipGetLastError());
+        }
+    }
+    return HPLAI_OK;
+}
+
+// x := U^-T x: U^T is lower with U's diagonal; element (r, c) = LU[r*lda + c]
+int launch_trsv_upper_t(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s, int nrhs,
+                        int64_t ldx)
+{
+    for (int64_t kb = 0; kb < n; kb += bs) {
+        const int w = n - kb < bs ? static_cast<int>(n - kb) : bs;
+        const float *Tkk = LU + kb * lda + kb;
+        launch_trsv_diag<false, false, true>(w, Tkk, lda, x + kb, ldx, nrhs, s);
+        HPLAI_HIP(hipGetLastError());
+        const int64_t m = n - kb - w;
+        if (m > 0) {
+            // rows kb + w + r', columns kb + c: LU[(kb + w + r') * lda + kb + c]
+            launch_gemv(true, m, w, LU + (kb + w) * lda + kb, lda, x + kb, x + kb + w, ldx, nrhs, s);
+            HPLAI_HIP(hipGetLastError());
+        }
+    }
+    return HPLAI_OK;
+}
+
+// x := L^-T x: L^T is unit upper; element (r, c) = LU[r*lda + c]
+int launch_trsv_lower_t(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s, int nrhs,
+                        int64_t ldx)
+{
+    const int64_t nblk = ceil_div(n, bs);
+    for (int64_t q = nblk - 1; q >= 0; --q) {
+        const int64_t kb = q * bs;
+        const int w = n - kb < bs ? static_cast<int>(n - kb) : bs;
+        const float *Tkk = LU + kb * lda + kb;
+        launch_trsv_diag<true, true, true>(w, Tkk, lda, x + kb, ldx, nrhs, s);
+        HPLAI_HIP(hipGetLastError());
+        if (kb > 0) {
+            // rows r < kb, columns kb + c: LU[r * lda + kb + c]
+            launch_gemv(true, kb, w, LU + kb, lda, x + kb, x, ldx, nrhs, s);
             HPLAI_HIP(hipGetLastError());
         }
     }

@@ -300,9 +300,134 @@ struct LU {
     }
 };
 
+// Fills the LU descriptor for an n x n matrix in B->A (leading dimension lda).
+void init_lu(LU &lu, hplai_ctx ctx, int64_t n, int64_t lda, int b, int fmt, int outer, bool lazy,
+             uint64_t seed, Buffers *B, GemmTimer *timer)
+{
+    const int es = fmt == HPLAI_FMT_E4M3 ? 1 : 2;
+    const int bk = 128 / es;
+    lu.timer = timer;
+    lu.n = n;
+    lu.lda = lda;
+    lu.b = b;
+    lu.outer = outer;
+    lu.fmt = fmt;
+    lu.es = es;
+    lu.kpad = (b + bk - 1) / bk * bk;
+    lu.ldq = static_cast<int64_t>(outer) * lu.kpad;
+    lu.ntiles = n / b;
+    lu.lazy = lazy;
+    lu.seed = seed;
+    lu.B = B;
+    lu.main = nullptr;
+    lu.side = ctx->side;
+    lu.ev = ctx->ev;
+    lu.num_cu = ctx->num_cu;
+}
+
+// Allocates the LU's working buffers: quantized panel double buffers, tile
+// exponents and maxima, diagonal inverses, pivot flags.  On failure the
+// buffers allocated so far stay in B for free_buffers.
+int alloc_lu_buffers(const LU &lu, Buffers &B, hipStream_t s)
+{
+    auto alloc = [](void **p, size_t bytes, const char *what) {
+        const hipError_t e = hipMalloc(p, bytes);
+        return e == hipSuccess ? HPLAI_OK : hip_fail(e, what);
+    };
+    const int64_t nt = lu.ntiles;
+    int rc = alloc(reinterpret_cast<void **>(&B.info), sizeof(int) * nt, "hipMalloc info");
+    if (rc == HPLAI_OK)
+        rc = alloc(reinterpret_cast<void **>(&B.inv), sizeof(float) * diag_inv_floats(lu.b) * nt, "hipMalloc inv");
+    for (int i = 0; i < 2 && rc == HPLAI_OK; ++i) {
+        const size_t qbytes = static_cast<size_t>(lu.n) * lu.ldq * lu.es;
+        rc = alloc(&B.Lq[i], qbytes, "hipMalloc Lq");
+        if (rc == HPLAI_OK) rc = alloc(&B.Uq[i], qbytes, "hipMalloc Uq");
+        if (rc == HPLAI_OK && hipMemsetAsync(B.Lq[i], 0, qbytes, s) != hipSuccess) rc = HPLAI_EHIP;
+        if (rc == HPLAI_OK && hipMemsetAsync(B.Uq[i], 0, qbytes, s) != hipSuccess) rc = HPLAI_EHIP;
+        if (rc == HPLAI_OK && lu.fmt != HPLAI_FMT_F16) {
+            const size_t cnt = static_cast<size_t>(lu.outer) * nt;
+            rc = alloc(reinterpret_cast<void **>(&B.eL[i]), sizeof(int32_t) * cnt, "hipMalloc eL");
+            if (rc == HPLAI_OK) rc = alloc(reinterpret_cast<void **>(&B.eU[i]), sizeof(int32_t) * cnt, "hipMalloc eU");
+            if (rc == HPLAI_OK) rc = alloc(reinterpret_cast<void **>(&B.amaxL[i]), sizeof(float) * cnt, "hipMalloc amaxL");
+            if (rc == HPLAI_OK) rc = alloc(reinterpret_cast<void **>(&B.amaxU[i]), sizeof(float) * cnt, "hipMalloc amaxU");
+        }
+    }
+    if (rc == HPLAI_OK && hipMemsetAsync(B.info, 0, sizeof(int) * nt, s) != hipSuccess) rc = HPLAI_EHIP;
+    return rc;
+}
+
+// After synchronisation: HPLAI_EPIVOT with the first zero / non-finite pivot.
+int check_pivots(const LU &lu, const Buffers &B, const char *who)
+{
+    std::vector<int> info(lu.ntiles);
+    HPLAI_HIP(hipMemcpy(info.data(), B.info, sizeof(int) * lu.ntiles, hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < lu.ntiles; ++q)
+        if (info[q] != 0) {
+            set_error("%s: zero or non-finite pivot at column %lld", who,
+                      (long long)(q * lu.b + info[q] - 1));
+            return HPLAI_EPIVOT;
+        }
+    return HPLAI_OK;
+}
+
+int check_lu_args(int64_t n, int b, int fmt, const char *who)
+{
+    HPLAI_CHECK(n > 0 && b > 0 && n % b == 0, "%s: need n %% b == 0 (n=%lld b=%d)", who, (long long)n, b);
+    HPLAI_CHECK(b <= 1024, "%s: b=%d > 1024", who, b);
+    HPLAI_CHECK(fmt == HPLAI_FMT_F16 || fmt == HPLAI_FMT_F16_TILE || fmt == HPLAI_FMT_E4M3,
+                "%s: unknown fmt %d", who, fmt);
+    if (fmt != HPLAI_FMT_F16)
+        HPLAI_CHECK(b % 128 == 0, "%s: tile-scaled formats need b %% 128 == 0 (b=%d)", who, b);
+    return HPLAI_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int hplai_factor_f32(hplai_ctx ctx, int64_t n, int b, int fmt, int outer, int lookahead, float *A,
+                     int64_t lda)
+{
+    HPLAI_CHECK(ctx != nullptr && A != nullptr, "hplai_factor_f32: NULL argument");
+    HPLAI_CHECK(lda >= n, "hplai_factor_f32: lda < n");
+    int rc = check_lu_args(n, b, fmt, "hplai_factor_f32");
+    if (rc) return rc;
+    HPLAI_HIP(hipSetDevice(ctx->device));
+    LU lu;
+    Buffers B;
+    GemmTimer timer;
+    init_lu(lu, ctx, n, lda, b, fmt, outer > 0 ? outer : 1, false, 0, &B, &timer);
+    B.A = A;   // the caller's matrix: not freed below
+    rc = alloc_lu_buffers(lu, B, nullptr);
+    if (rc == HPLAI_OK) rc = lu.run(lookahead != 0);
+    if (hipDeviceSynchronize() != hipSuccess && rc == HPLAI_OK) rc = hip_fail(hipGetLastError(), "hplai_factor_f32");
+    if (rc == HPLAI_OK) rc = check_pivots(lu, B, "hplai_factor_f32");
+    B.A = nullptr;
+    free_buffers(B, false);
+    return rc;
+}
+
+int hplai_lu_solve_f64(hplai_ctx ctx, int trans, int64_t n, int nrhs, int bs, const float *LU,
+                       int64_t lda, double *X, int64_t ldx, void *stream)
+{
+    HPLAI_CHECK(ctx != nullptr && LU != nullptr && X != nullptr, "hplai_lu_solve_f64: NULL argument");
+    HPLAI_CHECK(n >= 0 && nrhs >= 0 && bs > 0 && bs <= 1024 && lda >= n && (nrhs <= 1 || ldx >= n),
+                "hplai_lu_solve_f64: bad sizes (n=%lld nrhs=%d bs=%d lda=%lld ldx=%lld)", (long long)n,
+                nrhs, bs, (long long)lda, (long long)ldx);
+    HPLAI_CHECK(nrhs <= 65535, "hplai_lu_solve_f64: nrhs %d > 65535", nrhs);
+    if (n == 0 || nrhs == 0) return HPLAI_OK;
+    HPLAI_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc;
+    if (trans == 0) {
+        rc = launch_trsv_lower(n, bs, LU, lda, X, s, nrhs, ldx);
+        if (rc == HPLAI_OK) rc = launch_trsv_upper(n, bs, LU, lda, X, s, nrhs, ldx);
+    } else {
+        rc = launch_trsv_upper_t(n, bs, LU, lda, X, s, nrhs, ldx);
+        if (rc == HPLAI_OK) rc = launch_trsv_lower_t(n, bs, LU, lda, X, s, nrhs, ldx);
+    }
+    return rc;
+}
 
 int hplai_release(hplai_ctx ctx)
 {
@@ -345,41 +470,17 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     const int fmt = p->fmt;
     const int outer = p->outer > 0 ? p->outer : 1;
     const int maxit = p->maxit > 0 ? (p->maxit < 64 ? p->maxit : 64) : 50;
-    HPLAI_CHECK(n > 0 && b > 0 && n % b == 0, "hplai_solve: need n %% b == 0 (n=%lld b=%d)",
-                (long long)n, b);
-    HPLAI_CHECK(b <= 1024, "hplai_solve: b=%d > 1024", b);
-    HPLAI_CHECK(fmt == HPLAI_FMT_F16 || fmt == HPLAI_FMT_F16_TILE || fmt == HPLAI_FMT_E4M3,
-                "hplai_solve: unknown fmt %d", fmt);
-    const int es = fmt == HPLAI_FMT_E4M3 ? 1 : 2;
-    const int bk = 128 / es;
-    if (fmt != HPLAI_FMT_F16)
-        HPLAI_CHECK(b % 128 == 0, "hplai_solve: tile-scaled formats need b %% 128 == 0 (b=%d)", b);
+    int rc = check_lu_args(n, b, fmt, "hplai_solve");
+    if (rc) return rc;
     hplai_release(ctx);
 
     LU lu;
     Buffers B;
     GemmTimer timer;
     timer.on = p->profile != 0;
-    lu.timer = &timer;
-    lu.n = n;
-    lu.lda = (n + 63) / 64 * 64;
-    lu.b = b;
-    lu.outer = outer;
-    lu.fmt = fmt;
-    lu.es = es;
-    lu.kpad = (b + bk - 1) / bk * bk;
-    lu.ldq = static_cast<int64_t>(outer) * lu.kpad;
-    lu.ntiles = n / b;
-    lu.lazy = p->lazy != 0;
-    lu.seed = p->seed;
-    lu.B = &B;
-    lu.main = nullptr;
-    lu.side = ctx->side;
-    lu.ev = ctx->ev;
-    lu.num_cu = ctx->num_cu;
+    init_lu(lu, ctx, n, (n + 63) / 64 * 64, b, fmt, outer, p->lazy != 0, p->seed, &B, &timer);
     hipStream_t s = nullptr;
 
-    int rc = HPLAI_OK;
 #define ALLOC(ptr, bytes)                                                             \
     do {                                                                              \
         hipError_t _e = hipMalloc(reinterpret_cast<void **>(&(ptr)), (bytes));        \
@@ -396,24 +497,13 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     ALLOC(B.r, sizeof(double) * n);
     ALLOC(B.work, sizeof(double) * otf_gemv_work_doubles(n));
     ALLOC(B.norms, sizeof(double) * 4);
-    ALLOC(B.info, sizeof(int) * lu.ntiles);
     ALLOC(B.chk, sizeof(double) * 3);
-    ALLOC(B.inv, sizeof(float) * diag_inv_floats(b) * lu.ntiles);
-    for (int i = 0; i < 2; ++i) {
-        const size_t qbytes = static_cast<size_t>(n) * lu.ldq * es;
-        ALLOC(B.Lq[i], qbytes);
-        ALLOC(B.Uq[i], qbytes);
-        HPLAI_HIP(hipMemsetAsync(B.Lq[i], 0, qbytes, s));
-        HPLAI_HIP(hipMemsetAsync(B.Uq[i], 0, qbytes, s));
-        if (fmt != HPLAI_FMT_F16) {
-            ALLOC(B.eL[i], sizeof(int32_t) * outer * lu.ntiles);
-            ALLOC(B.eU[i], sizeof(int32_t) * outer * lu.ntiles);
-            ALLOC(B.amaxL[i], sizeof(float) * outer * lu.ntiles);
-            ALLOC(B.amaxU[i], sizeof(float) * outer * lu.ntiles);
-        }
-    }
 #undef ALLOC
-    HPLAI_HIP(hipMemsetAsync(B.info, 0, sizeof(int) * lu.ntiles, s));
+    rc = alloc_lu_buffers(lu, B, s);
+    if (rc) {
+        free_buffers(B, false);
+        return rc;
+    }
 
     // ---- generation (main.cpp:256-275), outside the timed region ----
     HPLAI_HIP(hipDeviceSynchronize());
@@ -437,17 +527,7 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     res->time_lu = t2 - t1;
     if (rc == HPLAI_OK) rc = timer.rc;
     timer.collect(res);
-    if (rc == HPLAI_OK) {
-        std::vector<int> info(lu.ntiles);
-        HPLAI_HIP(hipMemcpy(info.data(), B.info, sizeof(int) * lu.ntiles, hipMemcpyDeviceToHost));
-        for (int64_t q = 0; q < lu.ntiles; ++q)
-            if (info[q] != 0) {
-                set_error("hplai_solve: zero or non-finite pivot at column %lld",
-                          (long long)(q * b + info[q] - 1));
-                rc = HPLAI_EPIVOT;
-                break;
-            }
-    }
+    if (rc == HPLAI_OK) rc = check_pivots(lu, B, "hplai_solve");
 
     // ---- iterative refinement (iterative_refinement.hpp:18-57) ----
     int steps = 0;

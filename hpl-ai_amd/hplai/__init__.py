@@ -61,6 +61,8 @@ _SIGS = [
     ("hplai_trsv_lower_f32", _int, [_vp, _i64, _int, _vp, _i64, _vp, _vp]),
     ("hplai_trsv_upper_f32", _int, [_vp, _i64, _int, _vp, _i64, _vp, _vp]),
     ("hplai_norm_inf_f64", _int, [_vp, _i64, _vp, _vp, _vp]),
+    ("hplai_factor_f32", _int, [_vp, _i64, _int, _int, _int, _int, _vp, _i64]),
+    ("hplai_lu_solve_f64", _int, [_vp, _int, _i64, _int, _int, _vp, _i64, _vp, _i64, _vp]),
     ("hplai_solve", _int, [_vp, ctypes.POINTER(Params), ctypes.POINTER(Result)]),
     ("hplai_solution_x", _int, [_vp, _vp]),
     ("hplai_solution_factors", _int, [_vp, _vp, _i64]),
@@ -193,6 +195,17 @@ class Context:
 
     def norm_inf(self, n, x, out, stream=None):
         check(lib().hplai_norm_inf_f64(self.handle, n, ptr(x), ptr(out), stream_ptr(stream)))
+
+    # ---- dense factor and multi-RHS solves ----
+    def factor(self, n, b, A, lda, fmt="f16", outer=1, lookahead=True):
+        """LU of the n x n FP32 column-major device matrix A in place."""
+        code = FMTS[fmt] if isinstance(fmt, str) else fmt
+        check(lib().hplai_factor_f32(self.handle, n, b, code, outer, 1 if lookahead else 0, ptr(A), lda))
+
+    def lu_solve(self, n, nrhs, bs, LU, lda, X, ldx, trans=False, stream=None):
+        """X := (LU)^-1 X, or (LU)^-T X with trans; X FP64 with nrhs columns."""
+        check(lib().hplai_lu_solve_f64(self.handle, 1 if trans else 0, n, nrhs, bs, ptr(LU), lda,
+                                       ptr(X), ldx, stream_ptr(stream)))
 
     # ---- whole solve ----
     def solve(self, n, b, fmt="f16", outer=1, lookahead=True, lazy=False, seed=42, maxit=50,

@@ -122,6 +122,24 @@ int hplai_trsv_upper_f32(hplai_ctx ctx, int64_t n, int bs, const float *LU, int6
 int hplai_norm_inf_f64(hplai_ctx ctx, int64_t n, const double *x, double *out_dev,
                        void *stream);
 
+/* ---------------- dense factor and multi-RHS solves ---------------- */
+/* Replaces RDMAPanelLU::run (panel_trf.hpp:703-856) on a caller-supplied
+ * matrix: unpivoted LU of the n x n FP32 matrix A (device, column-major,
+ * lda >= n) in place, with the same kernels and schedule as hplai_solve
+ * (FP32 panel, Schur update in `fmt`, `outer` panels per trailing update,
+ * look-ahead on a second stream).  n % b == 0; b % 128 == 0 for the
+ * tile-scaled formats.  Synchronous; HPLAI_EPIVOT on a zero pivot. */
+int hplai_factor_f32(hplai_ctx ctx, int64_t n, int b, int fmt, int outer, int lookahead, float *A,
+                     int64_t lda);
+/* Solves with the factors of hplai_factor_f32 / hplai_solve (FP32 entries
+ * widened to FP64, as panel_trsvL/U panel_trsv.hpp:310-758 do):
+ * trans = 0: X := (L U)^-1 X;  trans = 1: X := (L U)^-T X = L^-T U^-T X,
+ * the transposed solve for A^T lambda = g (no reference counterpart: it serves
+ * the gradient of x = A^-1 b).  X (device, FP64) holds nrhs columns with
+ * leading dimension ldx; bs (<= 1024) is the blocking, normally the LU's b. */
+int hplai_lu_solve_f64(hplai_ctx ctx, int trans, int64_t n, int nrhs, int bs, const float *LU,
+                       int64_t lda, double *X, int64_t ldx, void *stream);
+
 /* ---------------- whole solve (main.cpp:256-452) ---------------- */
 typedef struct hplai_params {
     int64_t  n;          /* matrix order, n % b == 0                               */
