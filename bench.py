@@ -54,7 +54,8 @@ def run_form(ctx, args, fmt, outer):
     t = sum(times) / len(times)
     tot = {k: sum(r[k] for r in results) for k in
            ("schur_launches", "schur_flops", "schur_ms", "schur_large_launches", "schur_large_flops",
-            "schur_large_ms", "schur_large_bytes")}
+            "schur_large_ms", "schur_large_bytes", "schur_inner_launches", "schur_inner_flops",
+            "schur_inner_ms")}
     return dict(fmt=fmt, outer=outer, time_s=t, wall_s=wall, tflops=hpl_flops(args.n) / t * 1e-12,
                 time_lu_s=res["time_lu"], time_ir_s=res["time_ir"], time_gen_s=res["time_gen"],
                 ir_steps=res["ir_steps"], harness=res["harness"], residual=res["residual"],
@@ -88,7 +89,11 @@ def roofline(p, fmt, n):
             "kernel": "schur_dma_kernel<%s> (256x256 streamed Schur update)" % fmt,
             "launches": nl, "flops_per_launch": flops_per, "ms_per_launch": ms_per,
             "algorithmic_bytes_per_launch": p["schur_large_bytes"] / nl,
-            "all_schur_launches": {"launches": p["schur_launches"], "tflops": p["schur_tflops"]}}
+            "all_schur_launches": {"launches": p["schur_launches"], "tflops": p["schur_tflops"],
+                                   "ms": p["schur_ms"]},
+            "inner_k_b_launches": {"launches": p["schur_inner_launches"], "ms": p["schur_inner_ms"],
+                                   "tflops": (p["schur_inner_flops"] / (p["schur_inner_ms"] * 1e-3) * 1e-12)
+                                   if p["schur_inner_ms"] > 0 else None}}
 
 
 def cpu_baseline(n, b, threads):
@@ -108,8 +113,98 @@ def cpu_baseline(n, b, threads):
                        "%.2f s, harness %.4f" % (n, b, t, r["final_harness"]))
 
 
+def bench_batch(args):
+    """Batching form: 4096 seeded systems, sizes uniform in [32, 1024], 1-4
+    right-hand sides; autotuned panel widths; the persistent launch replayed
+    from a hipGraph against the one-launch-per-system loop."""
+    import hplai
+    ctx = hplai.Context(0)
+    n, nrhs, seeds = hplai.batch_spec(args.batch_count, seed=2024)
+    bt = hplai.Batch(ctx, n, nrhs, seeds)
+    table = bt.autotune()
+    for _ in range(max(args.warmup, 1)):
+        bt.run("graph")
+    runs = [bt.run("graph") for _ in range(args.steps)]
+    base = bt.run("per_system")
+    bt.close()
+    ctx.close()
+    t = sum(r["seconds"] for r in runs) / len(runs)
+    last = runs[-1]
+    line = {
+        "metric": "batched HPL-AI solves: systems/s, one persistent hipGraph launch, 1 GPU",
+        "value": args.batch_count / t, "unit": "systems/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": max(args.warmup, 1), "ms_per_step": t * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f16", "data": "synthetic (seeded LCG systems)",
+        "config": {"workload": "%d systems, n uniform in [32, 1024], 1-4 FP64 rhs" % args.batch_count,
+                   "autotuned_nb_per_bucket": table},
+        "p50_us": last["p50_us"], "p99_us": last["p99_us"], "max_us": last["max_us"],
+        "failed": last["failed"], "max_harness": last["max_harness"], "max_ir_steps": last["max_steps"],
+        "per_system_loop": {"systems_per_s": base["systems_per_s"], "seconds": base["seconds"],
+                            "p50_us": base["p50_us"], "p99_us": base["p99_us"], "failed": base["failed"]},
+        "speedup_vs_per_system_loop": base["seconds"] / t,
+    }
+    print(json.dumps(line))
+
+
+def bench_autograd(args):
+    """Differentiation form: X = solve(A, B), n = 8192, B 8192 x 64, loss
+    sum(W * X); forward and backward timed."""
+    import torch
+    import hplai
+    from hplai import autograd
+    n, k = 8192, 64
+    ctx = hplai.Context(0)
+    d = torch.empty(n, dtype=torch.float64, device="cuda")
+    ctx.gen_diag(n, 42, 0, n, d)
+    A32 = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    ctx.gen_matrix(n, 42, 0, 0, n, n, None, A32, n)
+    A = A32.t().double().contiguous()
+    A.diagonal().copy_(d)
+    g = torch.Generator(device="cpu").manual_seed(2024)
+    B = torch.randn((n, k), generator=g, dtype=torch.float64).cuda()
+    W = torch.randn((n, k), generator=g, dtype=torch.float64).cuda()
+    fw, bw, info = [], [], {}
+    for it in range(max(args.warmup, 1) + args.steps):
+        Ar = A.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        X = autograd.solve(Ar, B, ctx=ctx, b=512, fmt="f16", info=info)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        (W * X).sum().backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if it >= max(args.warmup, 1):
+            fw.append(t1 - t0)
+            bw.append(t2 - t1)
+    At = A.clone().requires_grad_(True)
+    (W * torch.linalg.solve(At, B)).sum().backward()
+    rel = float((Ar.grad - At.grad).abs().max() / At.grad.abs().max())
+    ctx.close()
+    tf, tb = sum(fw) / len(fw), sum(bw) / len(bw)
+    print(json.dumps({
+        "metric": "autograd solve X = A^-1 B, n=8192, 64 rhs: forward + backward time, 1 GPU",
+        "value": (tf + tb) * 1e3, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": max(args.warmup, 1),
+        "ms_per_step": (tf + tb) * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16", "data": "synthetic (seeded LCG matrix, N(0,1) B and W)",
+        "config": {"workload": "n=8192, B 8192x64 FP64, FP16 Schur LU b=512, FP64 refinement"},
+        "forward_ms": tf * 1e3, "backward_ms": tb * 1e3, "forward_ir_steps": info.get("forward_steps"),
+        "backward_ir_steps": info.get("backward_steps"), "dA_rel_err_vs_torch_linalg_solve": rel}))
+
+
+def bench_fp64(ctx, n):
+    """All-FP64 comparator at order n (see hplai.fp64_lu)."""
+    from hplai import fp64_lu
+    r = fp64_lu.fp64_solve(ctx, n)
+    x = r.pop("x").cpu()
+    return r, x
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--form", default="solve", choices=["solve", "batch", "autograd"],
+                    help="solve: the metric's workload (default); batch / autograd: the other forms")
+    ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
@@ -137,6 +232,10 @@ def main():
     import hplai
 
     torch.cuda.set_device(0)
+    if args.form == "batch":
+        return bench_batch(args)
+    if args.form == "autograd":
+        return bench_autograd(args)
     if args.copy_check_bytes > 0:
         src = torch.ones(args.copy_check_bytes // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

@@ -81,9 +81,9 @@ struct GemmTimer {
     bool on = false;
     std::vector<hipEvent_t> ev;   // start/stop pairs
     std::vector<double> flops, bytes;
-    std::vector<char> large;
+    std::vector<char> large, inner;
     int rc = HPLAI_OK;
-    void begin(hipStream_t s, double f, double by, bool is_large)
+    void begin(hipStream_t s, double f, double by, bool is_large, bool is_inner)
     {
         if (!on || rc) return;
         hipEvent_t a = nullptr, b = nullptr;
@@ -96,6 +96,7 @@ struct GemmTimer {
         flops.push_back(f);
         bytes.push_back(by);
         large.push_back(is_large ? 1 : 0);
+        inner.push_back(is_inner ? 1 : 0);
         (void)hipEventRecord(a, s);
     }
     void end(hipStream_t s)
@@ -106,8 +107,8 @@ struct GemmTimer {
     // after synchronisation
     void collect(hplai_result *r)
     {
-        double ms = 0.0, fl = 0.0, lms = 0.0, lfl = 0.0, lby = 0.0;
-        int64_t ln = 0;
+        double ms = 0.0, fl = 0.0, lms = 0.0, lfl = 0.0, lby = 0.0, ims = 0.0, ifl = 0.0;
+        int64_t ln = 0, in = 0;
         for (size_t i = 0; i + 1 < ev.size(); i += 2) {
             float t = 0.0f;
             if (hipEventElapsedTime(&t, ev[i], ev[i + 1]) != hipSuccess) t = 0.0f;
@@ -119,6 +120,11 @@ struct GemmTimer {
                 lfl += flops[i / 2];
                 lby += bytes[i / 2];
             }
+            if (inner[i / 2]) {
+                ++in;
+                ims += t;
+                ifl += flops[i / 2];
+            }
         }
         r->schur_launches = static_cast<int64_t>(flops.size());
         r->schur_flops = fl;
@@ -127,6 +133,9 @@ struct GemmTimer {
         r->schur_large_flops = lfl;
         r->schur_large_ms = lms;
         r->schur_large_bytes = lby;
+        r->schur_inner_launches = in;
+        r->schur_inner_flops = ifl;
+        r->schur_inner_ms = ims;
     }
     ~GemmTimer()
     {
@@ -181,7 +190,7 @@ struct LU {
         const double K = static_cast<double>(nsl) * b;
         timer->begin(s, 2.0 * static_cast<double>(m) * static_cast<double>(nn) * K,
                      static_cast<double>(m + nn) * K * es + 8.0 * static_cast<double>(m) * nn,
-                     schur_large_tiles(fmt, m, nn, b, num_cu));
+                     schur_large_tiles(fmt, m, nn, b, num_cu), nsl == 1 && outer > 1);
         const int rc = launch_schur(fmt, m, nn, nsl, kpad, lq(buf, row0, t0), ldq, el(buf, t0, row0),
                                     ntiles, uq(buf, col0, t0), ldq, eu(buf, t0, col0), ntiles, b,
                                     a(row0, col0), lda, num_cu, s);

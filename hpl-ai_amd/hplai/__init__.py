@@ -39,10 +39,17 @@ class Result(ctypes.Structure):
                 ("trace_resid", _dbl * 64), ("trace_harness", _dbl * 64), ("checksum", _dbl * 3),
                 ("schur_launches", _i64), ("schur_flops", _dbl), ("schur_ms", _dbl),
                 ("schur_large_launches", _i64), ("schur_large_flops", _dbl), ("schur_large_ms", _dbl),
-                ("schur_large_bytes", _dbl)]
+                ("schur_large_bytes", _dbl), ("schur_inner_launches", _i64), ("schur_inner_flops", _dbl),
+                ("schur_inner_ms", _dbl)]
 
 
 # (name, restype, argtypes) for every exported entry point of include/hplai.h
+class BatchStats(ctypes.Structure):
+    _fields_ = [("seconds", _dbl), ("systems_per_s", _dbl), ("p50_us", _dbl), ("p99_us", _dbl),
+                ("max_us", _dbl), ("failed", ctypes.c_int32), ("max_steps", ctypes.c_int32),
+                ("max_harness", _dbl)]
+
+
 _SIGS = [
     ("hplai_create", _int, [_int, ctypes.POINTER(_vp)]),
     ("hplai_destroy", _int, [_vp]),
@@ -63,6 +70,14 @@ _SIGS = [
     ("hplai_norm_inf_f64", _int, [_vp, _i64, _vp, _vp, _vp]),
     ("hplai_factor_f32", _int, [_vp, _i64, _int, _int, _int, _int, _vp, _i64]),
     ("hplai_lu_solve_f64", _int, [_vp, _int, _i64, _int, _int, _vp, _i64, _vp, _i64, _vp]),
+    ("hplai_batch_create", _int, [_vp, _int, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    ("hplai_batch_destroy", _int, [_vp]),
+    ("hplai_batch_set_nb", _int, [_vp, _vp]),
+    ("hplai_batch_get_nb", _int, [_vp, _vp]),
+    ("hplai_batch_autotune", _int, [_vp, _int, _vp]),
+    ("hplai_batch_run", _int, [_vp, _int, _int, ctypes.POINTER(BatchStats)]),
+    ("hplai_batch_results", _int, [_vp, _vp, _vp, _vp, _vp]),
+    ("hplai_batch_solution", _int, [_vp, _int, _vp]),
     ("hplai_solve", _int, [_vp, ctypes.POINTER(Params), ctypes.POINTER(Result)]),
     ("hplai_solution_x", _int, [_vp, _vp]),
     ("hplai_solution_factors", _int, [_vp, _vp, _i64]),
@@ -225,7 +240,9 @@ class Context:
                     trace_harness=list(r.trace_harness[:k]), checksum=list(r.checksum),
                     schur_launches=r.schur_launches, schur_flops=r.schur_flops, schur_ms=r.schur_ms,
                     schur_large_launches=r.schur_large_launches, schur_large_flops=r.schur_large_flops,
-                    schur_large_ms=r.schur_large_ms, schur_large_bytes=r.schur_large_bytes)
+                    schur_large_ms=r.schur_large_ms, schur_large_bytes=r.schur_large_bytes,
+                    schur_inner_launches=r.schur_inner_launches, schur_inner_flops=r.schur_inner_flops,
+                    schur_inner_ms=r.schur_inner_ms)
 
     def solution_x(self, n):
         import numpy as np
@@ -245,3 +262,85 @@ class Context:
 
 def exported_symbols():
     return [name for name, _, _ in _SIGS]
+
+
+# ---------------------------------------------------------------------------
+# batched solver
+# ---------------------------------------------------------------------------
+BATCH_MODES = {"persistent": 0, "graph": 1, "per_system": 2}
+
+
+def batch_spec(count=4096, seed=2024, nmin=32, nmax=1024, rhs_max=4):
+    """Seeded batch description: sizes uniform in [nmin, nmax], 1..rhs_max
+    right-hand sides, and one LCG seed per system."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    n = rng.integers(nmin, nmax + 1, size=count).astype(np.int32)
+    nrhs = rng.integers(1, rhs_max + 1, size=count).astype(np.int32)
+    seeds = rng.integers(1, 2 ** 62, size=count, dtype=np.int64).astype(np.uint64)
+    return n, nrhs, seeds
+
+
+class Batch:
+    """Many independent HPL-AI systems solved by one persistent launch
+    (hplai_batch_* in hplai.h)."""
+
+    def __init__(self, ctx, n, nrhs, seeds):
+        import numpy as np
+        self.ctx = ctx
+        self.n = np.ascontiguousarray(n, dtype=np.int32)
+        self.nrhs = np.ascontiguousarray(nrhs, dtype=np.int32)
+        self.seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        self.count = int(self.n.size)
+        h = _vp()
+        check(lib().hplai_batch_create(ctx.handle, self.count, self.n.ctypes.data, self.nrhs.ctypes.data,
+                                       self.seeds.ctypes.data, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().hplai_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_nb(self, nb):
+        arr = (ctypes.c_int32 * 4)(*nb)
+        check(lib().hplai_batch_set_nb(self.handle, arr))
+
+    def get_nb(self):
+        arr = (ctypes.c_int32 * 4)()
+        check(lib().hplai_batch_get_nb(self.handle, arr))
+        return list(arr)
+
+    def autotune(self, maxit=50):
+        arr = (ctypes.c_int32 * 4)()
+        check(lib().hplai_batch_autotune(self.handle, maxit, arr))
+        return list(arr)
+
+    def run(self, mode="persistent", maxit=50):
+        st = BatchStats()
+        check(lib().hplai_batch_run(self.handle, BATCH_MODES[mode], maxit, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in BatchStats._fields_}
+
+    def results(self):
+        import numpy as np
+        h = np.zeros((self.count, 4), dtype=np.float64)
+        st = np.zeros((self.count, 4), dtype=np.int32)
+        info = np.zeros(self.count, dtype=np.int32)
+        us = np.zeros(self.count, dtype=np.float64)
+        check(lib().hplai_batch_results(self.handle, h.ctypes.data, st.ctypes.data, info.ctypes.data,
+                                        us.ctypes.data))
+        return h, st, info, us
+
+    def solution(self, s):
+        """x of system s as an (n_s, nrhs_s) array."""
+        import numpy as np
+        x = np.zeros(int(self.n[s]) * int(self.nrhs[s]), dtype=np.float64)
+        check(lib().hplai_batch_solution(self.handle, int(s), x.ctypes.data))
+        return x.reshape(int(self.nrhs[s]), int(self.n[s])).T
+

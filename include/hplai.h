@@ -140,6 +140,45 @@ int hplai_factor_f32(hplai_ctx ctx, int64_t n, int b, int fmt, int outer, int lo
 int hplai_lu_solve_f64(hplai_ctx ctx, int trans, int64_t n, int nrhs, int bs, const float *LU,
                        int64_t lda, double *X, int64_t ldx, void *stream);
 
+/* ---------------- batched solver (no reference counterpart) ----------------
+ * Many independent HPL-AI systems (order 1..1024, 1..4 FP64 right-hand
+ * sides, each built by the reference generator from its own seed: element
+ * (i,j) = LCG state 1+i+n*j, diagonal = FP64 row sum of |a_ij|, rhs r =
+ * stream column n+r) are generated, factored (FP32 panels, FP16 Schur update
+ * with FP32 accumulation) and refined in FP64 to harness < 16 by one
+ * persistent launch: one workgroup per system, systems taken largest first
+ * from a device counter.  The LU panel width is chosen per size bucket
+ * ([1,128) [128,256) [256,512) [512,1024]) by hplai_batch_autotune. */
+typedef struct hplai_batch_s *hplai_batch;
+typedef struct hplai_batch_stats {
+    double  seconds;        /* host wall time of the run (launch to completion)      */
+    double  systems_per_s;
+    double  p50_us;         /* per-system device time (wall clock inside the kernel)  */
+    double  p99_us;
+    double  max_us;
+    int32_t failed;         /* systems with a zero pivot or a harness >= 16          */
+    int32_t max_steps;      /* most residual evaluations of any right-hand side      */
+    double  max_harness;
+} hplai_batch_stats;
+int hplai_batch_create(hplai_ctx ctx, int count, const int32_t *n, const int32_t *nrhs,
+                       const uint64_t *seed, hplai_batch *out);
+int hplai_batch_destroy(hplai_batch b);
+/* panel width (16, 32 or 64) per size bucket */
+int hplai_batch_set_nb(hplai_batch b, const int32_t nb[4]);
+int hplai_batch_get_nb(hplai_batch b, int32_t nb[4]);
+/* times every feasible panel width on up to 2 systems per CU of each bucket
+ * and keeps the fastest; nb_out (may be NULL) receives the table */
+int hplai_batch_autotune(hplai_batch b, int maxit, int32_t nb_out[4]);
+/* mode 0: one persistent launch; 1: the same launch captured once in a
+ * hipGraph and replayed; 2: one launch per system (baseline).  Synchronous. */
+int hplai_batch_run(hplai_batch b, int mode, int maxit, hplai_batch_stats *st);
+/* per system: harness[4 s + r], steps[4 s + r] (r < 4; -1 / 0 past nrhs),
+ * info[s], time_us[s]; any pointer may be NULL (host arrays) */
+int hplai_batch_results(hplai_batch b, double *harness, int32_t *steps, int32_t *info,
+                        double *time_us);
+/* solution of system s: n_s x nrhs_s doubles, column-major, to host memory */
+int hplai_batch_solution(hplai_batch b, int s, double *x_host);
+
 /* ---------------- whole solve (main.cpp:256-452) ---------------- */
 typedef struct hplai_params {
     int64_t  n;          /* matrix order, n % b == 0                               */
@@ -180,6 +219,10 @@ typedef struct hplai_result {
     double  schur_large_flops;
     double  schur_large_ms;
     double  schur_large_bytes;
+    /* the launches with K = b (updates inside the outer panel and its block row) */
+    int64_t schur_inner_launches;
+    double  schur_inner_flops;
+    double  schur_inner_ms;
 } hplai_result;
 
 /* Generate, factor and refine; synchronous.  Returns HPLAI_ENOCONV if the
