@@ -192,12 +192,43 @@ def bench_autograd(args):
         "backward_ir_steps": info.get("backward_steps"), "dA_rel_err_vs_torch_linalg_solve": rel}))
 
 
-def bench_fp64(ctx, n):
-    """All-FP64 comparator at order n (see hplai.fp64_lu)."""
+DBL_EPS = 2.220446049250313e-16
+
+
+def backward_error(harness, n):
+    """Normwise backward error ||b - A x|| / (||A|| ||x|| + ||b||) (inf-norms)
+    from the HPL-AI harness, which is that ratio divided by n eps / 2."""
+    return harness * n * DBL_EPS / 2 if harness is not None and harness >= 0 else None
+
+
+def fp64_comparison(ctx, args):
+    """The all-FP64 LU (hplai.fp64_lu) at the largest order that fits in HBM
+    in FP64 (n = 131072 by default: 137 GB), the mixed forms at the same n
+    with the forward error of their x against the FP64 x, and the FP64 time
+    extrapolated to args.n at the measured FP64 rate."""
+    import torch
     from hplai import fp64_lu
-    r = fp64_lu.fp64_solve(ctx, n)
-    x = r.pop("x").cpu()
-    return r, x
+    n = args.fp64_n
+    forms = {}
+    for fmt, outer in ((args.fmt, args.outer), (args.compare or None, args.compare_outer)):
+        if not fmt:
+            continue
+        r = ctx.solve(n, args.b, fmt=fmt, outer=outer, lookahead=not args.no_lookahead, keep_factors=True)
+        x = torch.from_numpy(ctx.solution_x(n))
+        ctx.release()
+        forms[fmt] = (r, x)
+    torch.cuda.empty_cache()
+    f = fp64_lu.fp64_solve(ctx, n)
+    x64 = f.pop("x").cpu()
+    torch.cuda.empty_cache()
+    out = {"n": n, "fp64_time_s": f["time_total"], "fp64_tflops": f["tflops"], "fp64_ir_steps": f["ir_steps"],
+           "fp64_backward_error": backward_error(f["harness"], n),
+           "fp64_time_extrapolated_to_n%d_s" % args.n: f["time_total"] * (args.n / n) ** 3}
+    for fmt, (r, x) in forms.items():
+        out[fmt] = {"time_s": r["time_total"], "speedup_vs_fp64": f["time_total"] / r["time_total"],
+                    "ir_steps": r["ir_steps"], "backward_error": backward_error(r["harness"], n),
+                    "forward_error_vs_fp64": float((x - x64).abs().max() / x64.abs().max())}
+    return out
 
 
 def main():
@@ -218,6 +249,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--cpu-b", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-n", type=int, default=131072,
+                    help="order of the all-FP64 LU comparison (0 = skip); n = 196608 needs 309 GB in FP64")
     ap.add_argument("--copy-check-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (calibrates "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE against a known traffic)")
@@ -252,6 +285,7 @@ def main():
     compare = None
     if args.compare:
         compare = run_form(ctx, args, args.compare, args.compare_outer)
+    fp64 = fp64_comparison(ctx, args) if args.fp64_n > 0 else None
     ctx.close()
 
     roof = roofline(primary, args.fmt, args.n)
@@ -285,13 +319,18 @@ def main():
         "time_ir_s": primary["time_ir_s"],
         "time_gen_s": primary["time_gen_s"],
         "wall_s_per_step": primary["wall_s"],
+        "backward_error": backward_error(primary["harness"], args.n),
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if fp64 is not None:
+        line["fp64_comparison"] = fp64
+        line["speedup_vs_fp64_extrapolated"] = fp64["fp64_time_extrapolated_to_n%d_s" % args.n] / primary["time_s"]
     if compare is not None:
         line["compare"] = {k: compare[k] for k in ("fmt", "outer", "time_s", "tflops", "ir_steps",
                                                    "harness", "time_lu_s", "time_ir_s",
                                                    "schur_tflops")}
+        line["compare"]["backward_error"] = backward_error(compare["harness"], args.n)
         line["speedup_vs_compare"] = compare["time_s"] / primary["time_s"]
     print(json.dumps(line))
 

@@ -43,7 +43,7 @@ constexpr int kBatchLds = 132 * 1024;     // panel / FP16 operands + L11 copy
 constexpr int kBuckets = 4;               // [32,128) [128,256) [256,512) [512,1024]
 
 struct BatchSys {
-    int32_t n, nrhs, nb, pad;
+    int32_t n, nrhs, nb, lda;             // lda = n rounded up to 4 (16-byte row groups)
     uint64_t x1;                          // lcg_init(seed)
     int64_t a32, a64, d, b, x, r;         // offsets into the float / double pools
 };
@@ -72,7 +72,7 @@ __device__ __forceinline__ double block_max(double v, double *red)
 // diagonal, d, and the right-hand sides (stream columns n .. n+nrhs-1) ----
 __device__ __noinline__ void batch_generate(const BatchSys &S, float *A32, double *A64, double *d, double *B)
 {
-    const int n = S.n, tid = threadIdx.x;
+    const int n = S.n, lda = S.lda, tid = threadIdx.x;
     const int64_t total = static_cast<int64_t>(n) * (n + S.nrhs);
     // element e (column-major, continuing into the rhs columns) is state 1 + e;
     // thread t takes e = t + 256 q, advancing 256 states per step
@@ -82,8 +82,9 @@ __device__ __noinline__ void batch_generate(const BatchSys &S, float *A32, doubl
     for (int64_t e = tid; e < total; e += kBT) {
         const double v = lcg_value(s);
         if (e < nn) {
-            A64[e] = v;
-            A32[e] = static_cast<float>(v);
+            const int64_t j = e / n, i = e - j * n;
+            A64[i + j * lda] = v;
+            A32[i + j * lda] = static_cast<float>(v);
         } else {
             B[e - nn] = v;
         }
@@ -94,10 +95,10 @@ __device__ __noinline__ void batch_generate(const BatchSys &S, float *A32, doubl
     for (int i = tid; i < n; i += kBT) {
         double sum = 0.0;
         for (int j = 0; j < n; ++j)
-            if (j != i) sum += fabs(A64[i + static_cast<int64_t>(j) * n]);
+            if (j != i) sum += fabs(A64[i + static_cast<int64_t>(j) * lda]);
         d[i] = sum;
-        A64[i + static_cast<int64_t>(i) * n] = sum;
-        A32[i + static_cast<int64_t>(i) * n] = static_cast<float>(sum);
+        A64[i + static_cast<int64_t>(i) * lda] = sum;
+        A32[i + static_cast<int64_t>(i) * lda] = static_cast<float>(sum);
     }
     __syncthreads();
 }
@@ -118,9 +119,10 @@ __host__ __device__ constexpr int64_t batch_lu_lds(int n, int nb)
            static_cast<int64_t>(nb) * nb * 4;
 }
 
-// ---- unpivoted LU of the n x n FP32 matrix A (lda n), panel width NB ----
+// ---- unpivoted LU of the n x n FP32 matrix A (leading dimension lda, a
+// multiple of 4), panel width NB ----
 template <int NB>
-__device__ __noinline__ int batch_lu(int n, float *A, uint8_t *lds)
+__device__ __noinline__ int batch_lu(int n, int lda, float *A, uint8_t *lds)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int RB = NB * 2;            // bytes of one FP16 K-row
@@ -132,7 +134,7 @@ __device__ __noinline__ int batch_lu(int n, float *A, uint8_t *lds)
         const int mp = n - k;
         // (a) panel into LDS
         for (int c = 0; c < w; ++c)
-            for (int r = tid; r < mp; r += kBT) P[c * mp + r] = A[(k + r) + static_cast<int64_t>(k + c) * n];
+            for (int r = tid; r < mp; r += kBT) P[c * mp + r] = A[(k + r) + static_cast<int64_t>(k + c) * lda];
         __syncthreads();
         // (a') unblocked LU of the tall panel (getrf_nopiv_small order per column)
         for (int c = 0; c < w; ++c) {
@@ -149,7 +151,7 @@ __device__ __noinline__ int batch_lu(int n, float *A, uint8_t *lds)
         }
         // (a'') factors back to A; keep L11 for the row block
         for (int c = 0; c < w; ++c)
-            for (int r = tid; r < mp; r += kBT) A[(k + r) + static_cast<int64_t>(k + c) * n] = P[c * mp + r];
+            for (int r = tid; r < mp; r += kBT) A[(k + r) + static_cast<int64_t>(k + c) * lda] = P[c * mp + r];
         for (int e = tid; e < w * w; e += kBT) L11[e] = P[(e / w) * mp + (e % w)];   // L11[c*w + r]
         const int m = n - k - w;
         if (m <= 0) break;
@@ -175,7 +177,7 @@ __device__ __noinline__ int batch_lu(int n, float *A, uint8_t *lds)
         for (int j = tid; j < mt; j += kBT) {
             float u[NB];
             if (j < m) {
-                const float *col = A + k + static_cast<int64_t>(k + w + j) * n;
+                const float *col = A + k + static_cast<int64_t>(k + w + j) * lda;
 #pragma unroll
                 for (int i = 0; i < NB; ++i) u[i] = i < w ? col[i] : 0.0f;
 #pragma unroll
@@ -183,7 +185,7 @@ __device__ __noinline__ int batch_lu(int n, float *A, uint8_t *lds)
 #pragma unroll
                     for (int t = 0; t < NB; ++t)
                         if (t < i && i < w) u[i] -= L11[t * w + i] * u[t];
-                float *colw = A + k + static_cast<int64_t>(k + w + j) * n;
+                float *colw = A + k + static_cast<int64_t>(k + w + j) * lda;
 #pragma unroll
                 for (int i = 0; i < NB; ++i)
                     if (i < w) colw[i] = u[i];
@@ -200,33 +202,59 @@ __device__ __noinline__ int batch_lu(int n, float *A, uint8_t *lds)
             }
         }
         __syncthreads();
-        // (c) trailing update: 32 x 32 tiles, one per wave at a time
+        // (c) trailing update: 32 x 32 tiles, four per wave at a time so that
+        // sixteen 16-byte loads per lane are in flight; rows go in aligned
+        // groups of 4 (lda % 4 == 0, so a group never leaves the allocation)
         const int T = mt / 32;
         const int lrow = lane & 31, lg = lane >> 5;
-        for (int t = wave; t < T * T; t += 4) {
-            const int ti = t % T, tj = t / T;
-            floatx16b_t acc;
+        for (int t0 = wave * 4; t0 < T * T; t0 += 16) {
+            float4 cv[4][4];
 #pragma unroll
-            for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+            for (int u = 0; u < 4; ++u) {
+                const int t = t0 + u;
+                const int ti = t % T, tj = t / T;
+                const int gc = tj * 32 + lrow;
+                const float *ccol = A + (k + w) + static_cast<int64_t>(k + w + (gc < m ? gc : 0)) * lda;
 #pragma unroll
-            for (int kk = 0; kk < NB / 16; ++kk) {
-                const int ra = ti * 32 + lrow, rb = tj * 32 + lrow;
-                const int ch = kk * 2 + lg;                 // 16-byte chunk: halves [8 ch, 8 ch + 8)
-                const half8b_t a = *reinterpret_cast<const half8b_t *>(Lh + ra * RB + chunk_pos<NB>(ra, ch) * 16);
-                const half8b_t b = *reinterpret_cast<const half8b_t *>(Uh + rb * RB + chunk_pos<NB>(rb, ch) * 16);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
+                for (int q = 0; q < 4; ++q) {
+                    const int gr = ti * 32 + 8 * q + 4 * lg;
+                    cv[u][q] = (t < T * T && gc < m && gr < m) ? *reinterpret_cast<const float4 *>(ccol + gr)
+                                                               : float4{0.0f, 0.0f, 0.0f, 0.0f};
+                }
             }
-            // acc[4 q + r] is row 8 q + 4 lg + r, column lrow of the tile
-            const int gc = tj * 32 + lrow;
-            if (gc < m) {
-                float *ccol = A + (k + w) + static_cast<int64_t>(k + w + gc) * n;
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
+            for (int u = 0; u < 4; ++u) {
+                const int t = t0 + u;
+                if (t >= T * T) break;
+                const int ti = t % T, tj = t / T;
+                floatx16b_t acc;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int gr = ti * 32 + 8 * q + 4 * lg + r;
-                        if (gr < m) ccol[gr] -= acc[4 * q + r];
+                for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+#pragma unroll
+                for (int kk = 0; kk < NB / 16; ++kk) {
+                    const int ra = ti * 32 + lrow, rb = tj * 32 + lrow;
+                    const int ch = kk * 2 + lg;             // 16-byte chunk: halves [8 ch, 8 ch + 8)
+                    const half8b_t a = *reinterpret_cast<const half8b_t *>(Lh + ra * RB + chunk_pos<NB>(ra, ch) * 16);
+                    const half8b_t b = *reinterpret_cast<const half8b_t *>(Uh + rb * RB + chunk_pos<NB>(rb, ch) * 16);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
+                }
+                // acc[4 q + r] is row 8 q + 4 lg + r, column lrow of the tile
+                const int gc = tj * 32 + lrow;
+                if (gc < m) {
+                    float *ccol = A + (k + w) + static_cast<int64_t>(k + w + gc) * lda;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int gr = ti * 32 + 8 * q + 4 * lg;
+                        if (gr < m) {
+                            float4 c4 = cv[u][q];
+                            c4.x -= acc[4 * q + 0];
+                            c4.y -= acc[4 * q + 1];
+                            c4.z -= acc[4 * q + 2];
+                            c4.w -= acc[4 * q + 3];
+                            *reinterpret_cast<float4 *>(ccol + gr) = c4;
+                        }
                     }
+                }
             }
         }
         __syncthreads();
@@ -237,20 +265,20 @@ __device__ __noinline__ int batch_lu(int n, float *A, uint8_t *lds)
 
 // each size class in its own function, so each gets its own register budget
 template <int RPT, bool UPPER, bool UNIT>
-__device__ __noinline__ void batch_trsv_rpt(int n, const float *A, double *x, double *xs)
+__device__ __noinline__ void batch_trsv_rpt(int n, int lda, const float *A, double *x, double *xs)
 {
-    trsv_block<RPT, UPPER, UNIT, false>(n, A, n, x, xs);
+    trsv_block<RPT, UPPER, UNIT, false>(n, A, lda, x, xs);
 }
 
 template <bool UPPER, bool UNIT>
-__device__ void batch_trsv(int n, const float *A, double *x, double *xs)
+__device__ void batch_trsv(int n, int lda, const float *A, double *x, double *xs)
 {
     if (n <= 256)
-        batch_trsv_rpt<1, UPPER, UNIT>(n, A, x, xs);
+        batch_trsv_rpt<1, UPPER, UNIT>(n, lda, A, x, xs);
     else if (n <= 512)
-        batch_trsv_rpt<2, UPPER, UNIT>(n, A, x, xs);
+        batch_trsv_rpt<2, UPPER, UNIT>(n, lda, A, x, xs);
     else
-        batch_trsv_rpt<4, UPPER, UNIT>(n, A, x, xs);
+        batch_trsv_rpt<4, UPPER, UNIT>(n, lda, A, x, xs);
 }
 
 // ---- FP64 refinement of every right-hand side (iterative_refinement.hpp:18-57) ----
@@ -258,7 +286,7 @@ __device__ __noinline__ void batch_refine(const BatchSys &S, const float *LU, co
                              const double *B, double *X, double *R, double *xl, double *xs, double *red,
                              BatchOut &o, int maxit)
 {
-    const int n = S.n, tid = threadIdx.x;
+    const int n = S.n, lda = S.lda, tid = threadIdx.x;
     double dm = 0.0;
     for (int i = tid; i < n; i += kBT) dm = fmax(dm, fabs(d[i]));
     const double norma = 2.0 * block_max(dm, red);          // hpl_infnorm, panel_norm.hpp:78-84
@@ -280,7 +308,7 @@ __device__ __noinline__ void batch_refine(const BatchSys &S, const float *LU, co
             double xm = 0.0, rm = 0.0;
             for (int i = tid; i < n; i += kBT) {
                 double acc = b[i];
-                for (int j = 0; j < n; ++j) acc = fma(-A64[i + static_cast<int64_t>(j) * n], xl[j], acc);
+                for (int j = 0; j < n; ++j) acc = fma(-A64[i + static_cast<int64_t>(j) * lda], xl[j], acc);
                 R[i] = acc;
                 rm = fmax(rm, fabs(acc));
                 xm = fmax(xm, fabs(xl[i]));
@@ -290,8 +318,8 @@ __device__ __noinline__ void batch_refine(const BatchSys &S, const float *LU, co
             h = normr / (norma * normx + normb) * 1. / (static_cast<double>(n) * DBL_EPSILON / 2);
             steps = it + 1;
             if (h < 16.0) break;
-            batch_trsv<false, true>(n, LU, R, xs);
-            batch_trsv<true, false>(n, LU, R, xs);
+            batch_trsv<false, true>(n, lda, LU, R, xs);
+            batch_trsv<true, false>(n, lda, LU, R, xs);
             for (int i = tid; i < n; i += kBT) x[i] += R[i];
             __syncthreads();
         }
@@ -327,11 +355,11 @@ __global__ __launch_bounds__(kBT) void batch_kernel(const BatchSys *__restrict__
         batch_generate(S, A32, A64, d, B);
         int info;
         if (S.nb == 16)
-            info = batch_lu<16>(S.n, A32, lds);
+            info = batch_lu<16>(S.n, S.lda, A32, lds);
         else if (S.nb == 64)
-            info = batch_lu<64>(S.n, A32, lds);
+            info = batch_lu<64>(S.n, S.lda, A32, lds);
         else
-            info = batch_lu<32>(S.n, A32, lds);
+            info = batch_lu<32>(S.n, S.lda, A32, lds);
         BatchOut o;
         o.info = info;
         if (info == 0)
@@ -432,12 +460,12 @@ int hplai_batch_create(hplai_ctx ctx, int count, const int32_t *n, const int32_t
     int64_t fo = 0, dof = 0;
     auto up = [](int64_t v) { return (v + 63) / 64 * 64; };
     for (int s = 0; s < count; ++s) {
-        const int64_t nn = static_cast<int64_t>(n[s]) * n[s];
         BatchSys &S = b->sys[s];
         S.n = n[s];
         S.nrhs = nrhs[s];
         S.nb = 32;
-        S.pad = 0;
+        S.lda = (n[s] + 3) / 4 * 4;
+        const int64_t nn = static_cast<int64_t>(S.lda) * n[s];
         S.x1 = lcg_init(seed[s]);
         S.a32 = fo;
         fo += up(nn);

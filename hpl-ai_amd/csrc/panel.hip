@@ -171,6 +171,138 @@ __global__ __launch_bounds__(kGetrfThreads) void getrf_kernel(int n, float *__re
 }
 
 // ---------------------------------------------------------------------------
+// The same blocked LU for n <= 512 with n % 32 == 0 and 16-byte aligned
+// columns, with the trailing update on v_mfma_f32_32x32x2_f32 (FP32 inputs,
+// FP32 accumulation): each wave updates 32 x 32 tiles of A22, four tiles at a
+// time with their sixteen float4 loads per lane in flight together.  L21 and
+// U12 come from LDS (column-major L21, row-major U12).
+// ---------------------------------------------------------------------------
+typedef float floatx16g_t __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(kGetrfThreads) void getrf_mfma_kernel(int n, float *__restrict__ A, int64_t lda,
+                                                                   int *__restrict__ info)
+{
+    constexpr int NT = kGetrfThreads, NMAX = 512, W = 32, RPT = NMAX / NT;
+    __shared__ __attribute__((aligned(16))) float Lc[W * NMAX];   // panel, Lc[c * NMAX + r]
+    __shared__ __attribute__((aligned(16))) float Us[W * NMAX];   // U strip, Us[r * NMAX + c]
+    __shared__ float prow[2][W];
+    __shared__ float Mi[32 * 33], Ti[32 * 33];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, lh = lane >> 5;
+    int bad = 0;
+    for (int k = 0; k < n; k += W) {
+        const int h = n - k;
+        float *Ak = A + static_cast<int64_t>(k) * lda + k;
+        float p[RPT][W];
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = tid + q * NT;
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                const float v = Ak[static_cast<int64_t>(c) * lda + min(r, h - 1)];
+                p[q][c] = r < h ? v : 0.0f;
+            }
+        }
+        // unblocked LU of the h x 32 panel, one barrier per pivot
+#pragma unroll
+        for (int kk = 0; kk < W; ++kk) {
+            if (tid == kk)
+#pragma unroll
+                for (int c = 0; c < W; ++c) prow[kk & 1][c] = p[0][c];
+            __syncthreads();
+            const float piv = prow[kk & 1][kk];
+            const float inv = 1.0f / piv;
+            if (tid == 0 && bad == 0 && !(piv != 0.0f && isfinite(piv))) bad = k + kk + 1;
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int r = tid + q * NT;
+                if (r > kk && r < h) {
+                    const float l = p[q][kk] * inv;
+                    p[q][kk] = l;
+#pragma unroll
+                    for (int c = kk + 1; c < W; ++c) p[q][c] = p[q][c] - l * prow[kk & 1][c];
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = tid + q * NT;
+            if (r < h)
+#pragma unroll
+                for (int c = 0; c < W; ++c) {
+                    Ak[static_cast<int64_t>(c) * lda + r] = p[q][c];
+                    Lc[c * NMAX + r] = p[q][c];
+                }
+        }
+        __syncthreads();
+        const int wr = n - k - W;
+        if (wr <= 0) break;
+        // U strip: x := L11^-1 x through the inverse of the unit-lower L11
+        for (int e = tid; e < 32 * 32; e += NT) {
+            const int r = e / 32, c = e % 32;
+            Mi[r * 33 + c] = r > c ? Lc[c * NMAX + r] : (r == c ? 1.0f : 0.0f);
+        }
+        __syncthreads();
+        lower_inv32(Mi, Ti, tid, NT);
+        for (int c = tid; c < wr; c += NT) {
+            float x[W];
+            float *col = Ak + static_cast<int64_t>(W + c) * lda;
+#pragma unroll
+            for (int r = 0; r < W; ++r) x[r] = col[r];
+            for (int i = 0; i < W; ++i) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int r = 0; r < W; ++r) acc = fmaf(Mi[i * 33 + r], x[r], acc);
+                col[i] = acc;
+                Us[i * NMAX + c] = acc;
+            }
+        }
+        __syncthreads();
+        // trailing update A22 -= L21 U12 (hr x wr, both multiples of 32)
+        const int TR = (h - W) / 32, TC = wr / 32;
+        for (int t0 = wave * 4; t0 < TR * TC; t0 += 4 * (NT / 64)) {
+            float4 cv[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int t = t0 + u < TR * TC ? t0 + u : TR * TC - 1;
+                const int ti = t % TR, tj = t / TR;
+                const float *ccol = Ak + static_cast<int64_t>(W + tj * 32 + l32) * lda + W + ti * 32 + 4 * lh;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) cv[u][q] = *reinterpret_cast<const float4 *>(ccol + 8 * q);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int t = t0 + u;
+                if (t >= TR * TC) break;
+                const int ti = t % TR, tj = t / TR;
+                floatx16g_t acc;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk) {
+                    const int kr = 2 * kk + lh;
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Lc[kr * NMAX + W + ti * 32 + l32],
+                                                               Us[kr * NMAX + tj * 32 + l32], acc, 0, 0, 0);
+                }
+                float *ccol = Ak + static_cast<int64_t>(W + tj * 32 + l32) * lda + W + ti * 32 + 4 * lh;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    float4 c4 = cv[u][q];
+                    c4.x -= acc[4 * q + 0];
+                    c4.y -= acc[4 * q + 1];
+                    c4.z -= acc[4 * q + 2];
+                    c4.w -= acc[4 * q + 3];
+                    *reinterpret_cast<float4 *>(ccol + 8 * q) = c4;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && info != nullptr) *info = bad;
+}
+
+// ---------------------------------------------------------------------------
 // L panel TRSM: L(m x b) := L U^-1, U upper non-unit (update_left_panels,
 // panel_trf.hpp:143-157 -> strsm "R","U","N","N").  One thread per row; the
 // row is solved in column blocks of CB held in registers; each block's
@@ -394,7 +526,10 @@ __global__ __launch_bounds__(256) void quant_u_kernel(int64_t m, int64_t k, int 
 int launch_getrf(int n, float *A, int64_t lda, int *info, hipStream_t s)
 {
     if (n <= 0) return HPLAI_OK;
-    if (n <= 512)
+    const bool mfma_ok = n % 32 == 0 && lda % 4 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0;
+    if (n <= 512 && mfma_ok)
+        hipLaunchKernelGGL(getrf_mfma_kernel, dim3(1), dim3(kGetrfThreads), 0, s, n, A, lda, info);
+    else if (n <= 512)
         hipLaunchKernelGGL((getrf_kernel<512, 32>), dim3(1), dim3(kGetrfThreads), 0, s, n, A, lda, info);
     else
         hipLaunchKernelGGL((getrf_kernel<1024, 16>), dim3(1), dim3(kGetrfThreads), 0, s, n, A, lda, info);

@@ -53,6 +53,61 @@ __device__ __forceinline__ int xcd_remap(int w, int nwg)
     return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
 
+// C -= acc for JB accumulator blocks side by side (columns col0 + 32 j ..),
+// rows row0 .. row0 + 31.  acc[j][4 q + r] is row 8 q + 4 lgrp + r, column
+// lrow of block j.  With VEC_C every lane loads its 4 JB float4 first
+// (addresses outside C are redirected to C itself and the values dropped),
+// so the loads are in flight together; a row group cut by M (M % 4 != 0)
+// takes a scalar path.
+template <int JB, bool VEC_C>
+__device__ __forceinline__ void epilogue_block_row(float *__restrict__ C, int64_t ldc, int64_t M, int64_t N,
+                                                   int64_t row0, int64_t col0, int lrow, int lgrp,
+                                                   const floatx16_t (&acc)[JB])
+{
+    if constexpr (VEC_C) {
+        float4 cv[JB][4];
+#pragma unroll
+        for (int j = 0; j < JB; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int64_t gc = col0 + j * 32 + lrow, gr = row0 + 8 * q + 4 * lgrp;
+                const bool full = gc < N && gr + 3 < M;
+                cv[j][q] = *reinterpret_cast<const float4 *>(full ? C + gc * ldc + gr : C);
+            }
+#pragma unroll
+        for (int j = 0; j < JB; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int64_t gc = col0 + j * 32 + lrow, gr = row0 + 8 * q + 4 * lgrp;
+                if (gc < N && gr + 3 < M) {
+                    float4 c4 = cv[j][q];
+                    c4.x -= acc[j][4 * q + 0];
+                    c4.y -= acc[j][4 * q + 1];
+                    c4.z -= acc[j][4 * q + 2];
+                    c4.w -= acc[j][4 * q + 3];
+                    *reinterpret_cast<float4 *>(C + gc * ldc + gr) = c4;
+                } else if (gc < N) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (gr + r < M) C[gc * ldc + gr + r] -= acc[j][4 * q + r];
+                }
+            }
+    } else {
+#pragma unroll
+        for (int j = 0; j < JB; ++j) {
+            const int64_t gc = col0 + j * 32 + lrow;
+            if (gc >= N) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int64_t gr = row0 + 8 * q + 4 * lgrp;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (gr + r < M) C[gc * ldc + gr + r] -= acc[j][4 * q + r];
+            }
+        }
+    }
+}
+
 template <int FMT, int BM, int BN, bool VEC_C>
 __global__ __launch_bounds__(kThreads) void schur_kernel(
     int64_t M, int64_t N, int nslices, int kslice, const uint8_t *__restrict__ Lq, int64_t ldl_b,
@@ -224,36 +279,18 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
 
     // ---- epilogue: C -= acc (FP16 tile mode: acc * 2^-e_cur) ----
     // acc[i][j][4*q + r] holds row 8q + 4*lgrp + r, column lrow of block (i, j)
+    if (FMT == HPLAI_FMT_F16_TILE) {
+#pragma unroll
+        for (int i = 0; i < IB; ++i)
+#pragma unroll
+            for (int j = 0; j < JB; ++j)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[i][j][r] = ldexp_fast(acc[i][j][r], -e_cur);
+    }
 #pragma unroll
     for (int i = 0; i < IB; ++i)
-#pragma unroll
-        for (int j = 0; j < JB; ++j) {
-            const int64_t gc = n0 + wn * (BN / 2) + j * 32 + lrow;
-            if (gc >= N) continue;
-            float *ccol = C + gc * ldc;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int64_t gr = m0 + wm * (BM / 2) + i * 32 + 8 * q + 4 * lgrp;
-                float v[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    v[r] = (FMT == HPLAI_FMT_F16_TILE) ? ldexp_fast(acc[i][j][4 * q + r], -e_cur)
-                                                        : acc[i][j][4 * q + r];
-                if (VEC_C && gr + 3 < M) {
-                    float4 *p = reinterpret_cast<float4 *>(ccol + gr);
-                    float4 c4 = *p;
-                    c4.x -= v[0];
-                    c4.y -= v[1];
-                    c4.z -= v[2];
-                    c4.w -= v[3];
-                    *p = c4;
-                } else {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        if (gr + r < M) ccol[gr + r] -= v[r];
-                }
-            }
-        }
+        epilogue_block_row<JB, VEC_C>(C, ldc, M, N, m0 + wm * (BM / 2) + i * 32, n0 + wn * (BN / 2), lrow, lgrp,
+                                      acc[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -479,32 +516,12 @@ __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
     // drain the clamped tail loads before the workgroup can exit
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 
-    // ---- epilogue: C -= acc ----
+    // ---- epilogue: C -= acc, one accumulator row block (16 float4 per
+    // lane) at a time with all its loads in flight together ----
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t gc = n0 + wn * (BN / 2) + j * 32 + lrow;
-            if (gc >= N) continue;
-            float *ccol = C + gc * ldc;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int64_t gr = m0 + wm * (BM / 2) + i * 32 + 8 * q + 4 * lgrp;
-                if (VEC_C && gr + 3 < M) {
-                    float4 *p = reinterpret_cast<float4 *>(ccol + gr);
-                    float4 c4 = *p;
-                    c4.x -= acc[i][j][4 * q + 0];
-                    c4.y -= acc[i][j][4 * q + 1];
-                    c4.z -= acc[i][j][4 * q + 2];
-                    c4.w -= acc[i][j][4 * q + 3];
-                    *p = c4;
-                } else {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        if (gr + r < M) ccol[gr + r] -= acc[i][j][4 * q + r];
-                }
-            }
-        }
+        epilogue_block_row<4, VEC_C>(C, ldc, M, N, m0 + wm * (BM / 2) + i * 32, n0 + wn * (BN / 2), lrow, lgrp,
+                                     acc[i]);
 }
 
 template <int FMT>

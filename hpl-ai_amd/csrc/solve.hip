@@ -77,12 +77,36 @@ double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Per-launch HIP events for the Schur updates (roofline accounting) and per
+// phase of the reference's Timer (timer.hpp:58-237); only when profiling.
 struct GemmTimer {
     bool on = false;
     std::vector<hipEvent_t> ev;   // start/stop pairs
     std::vector<double> flops, bytes;
     std::vector<char> large, inner;
+    std::vector<hipEvent_t> pev;  // phase start/stop pairs
+    std::vector<int> pitem;
+    std::vector<double> pops;
     int rc = HPLAI_OK;
+    void pbegin(hipStream_t s)
+    {
+        if (!on || rc) return;
+        hipEvent_t a = nullptr, b = nullptr;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            rc = HPLAI_EHIP;
+            return;
+        }
+        pev.push_back(a);
+        pev.push_back(b);
+        (void)hipEventRecord(a, s);
+    }
+    void pend(hipStream_t s, int item, double ops)
+    {
+        if (!on || rc || pev.empty()) return;
+        (void)hipEventRecord(pev.back(), s);
+        pitem.push_back(item);
+        pops.push_back(ops);
+    }
     void begin(hipStream_t s, double f, double by, bool is_large, bool is_inner)
     {
         if (!on || rc) return;
@@ -105,8 +129,10 @@ struct GemmTimer {
         (void)hipEventRecord(ev.back(), s);
     }
     // after synchronisation
+    // after synchronisation; recomputes every field, so it can be called again
     void collect(hplai_result *r)
     {
+        for (int i = 0; i < HPLAI_PHASES; ++i) r->phase_seconds[i] = r->phase_ops[i] = 0.0;
         double ms = 0.0, fl = 0.0, lms = 0.0, lfl = 0.0, lby = 0.0, ims = 0.0, ifl = 0.0;
         int64_t ln = 0, in = 0;
         for (size_t i = 0; i + 1 < ev.size(); i += 2) {
@@ -136,9 +162,69 @@ struct GemmTimer {
         r->schur_inner_launches = in;
         r->schur_inner_flops = ifl;
         r->schur_inner_ms = ims;
+        for (size_t i = 0; i < ev.size() / 2; ++i) {
+            float t = 0.0f;
+            if (hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]) != hipSuccess) t = 0.0f;
+            r->phase_seconds[HPLAI_PHASE_GEMM_UPDATE] += 1e-3 * t;
+            r->phase_ops[HPLAI_PHASE_GEMM_UPDATE] += flops[i];
+        }
+        for (size_t i = 0; i < pitem.size(); ++i) {
+            float t = 0.0f;
+            if (hipEventElapsedTime(&t, pev[2 * i], pev[2 * i + 1]) != hipSuccess) t = 0.0f;
+            r->phase_seconds[pitem[i]] += 1e-3 * t;
+            r->phase_ops[pitem[i]] += pops[i];
+        }
     }
     ~GemmTimer()
     {
+        for (auto e : ev) (void)hipEventDestroy(e);
+        for (auto e : pev) (void)hipEventDestroy(e);
+    }
+};
+
+// "!epoch" progress lines (panel_trf.hpp:730-745): an event on the main stream
+// at the start of every outer step; a line is printed once its event has
+// completed (checked without blocking at each step, all at the end), with
+// the elapsed GPU time since the LU started.  One GPU is a 1 x 1 grid, so the
+// reference's epoch is one panel of b columns; lines come at outer steps.
+struct EpochLog {
+    bool on = false;
+    int64_t n = 0;
+    int b = 1;
+    hipEvent_t t0 = nullptr;
+    std::vector<hipEvent_t> ev;
+    std::vector<int64_t> col;
+    size_t printed = 0;
+    void start(hipStream_t s)
+    {
+        if (on && hipEventCreate(&t0) == hipSuccess) (void)hipEventRecord(t0, s);
+    }
+    void mark(hipStream_t s, int64_t c)
+    {
+        if (!on || t0 == nullptr) return;
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        (void)hipEventRecord(e, s);
+        ev.push_back(e);
+        col.push_back(c);
+    }
+    void flush(bool wait)
+    {
+        for (; printed < ev.size(); ++printed) {
+            if (!wait && hipEventQuery(ev[printed]) != hipSuccess) break;
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, t0, ev[printed]) != hipSuccess) ms = 0.0f;
+            const double el = 1e-3 * ms;
+            const double dn = static_cast<double>(n), nrem = static_cast<double>(n - col[printed]);
+            const double flop = 2. / 3. * (dn * dn * dn - nrem * nrem * nrem);
+            std::printf("!epoch %lld/%lld: elapsed=%f, %f Pflops (estimate)\n", (long long)(col[printed] / b),
+                        (long long)(n / b), el, el > 0 ? 1.e-15 * flop / el : 0.0);
+            std::fflush(stdout);
+        }
+    }
+    ~EpochLog()
+    {
+        if (t0) (void)hipEventDestroy(t0);
         for (auto e : ev) (void)hipEventDestroy(e);
     }
 };
@@ -154,6 +240,7 @@ struct LU {
     hipEvent_t *ev;
     int num_cu;
     GemmTimer *timer;
+    EpochLog *epochs = nullptr;
 
     bool tiled() const { return fmt != HPLAI_FMT_F16; }
     float *a(int64_t i, int64_t j) const { return B->A + j * lda + i; }
@@ -198,12 +285,24 @@ struct LU {
         return rc;
     }
 
+    // one phase of the reference's Timer around the launches of f
+    template <typename F>
+    int timed(int item, double ops, hipStream_t s, F &&f) const
+    {
+        timer->pbegin(s);
+        const int rc = f();
+        timer->pend(s, item, ops);
+        return rc;
+    }
+
     // Lazy initialisation (-d): add the generated a_ij to the partial sums of
     // an m x k block when it joins a panel (LazyInitializer, lazy_init.hpp:92-227).
     int lazy_init(int64_t i0, int64_t j0, int64_t m, int64_t k, hipStream_t s) const
     {
         if (!lazy || m <= 0 || k <= 0) return HPLAI_OK;
-        return launch_gen_matrix(n, seed, i0, j0, m, k, B->d, a(i0, j0), lda, 1, s);
+        return timed(HPLAI_PHASE_LAZY_INIT, static_cast<double>(m) * k, s, [&] {
+            return launch_gen_matrix(n, seed, i0, j0, m, k, B->d, a(i0, j0), lda, 1, s);
+        });
     }
 
     // Step 1 for the outer block at c0 with sw inner blocks.
@@ -218,26 +317,39 @@ struct LU {
         }
         for (int t = 0; t < sw && rc == HPLAI_OK; ++t) {
             const int64_t c = c0 + static_cast<int64_t>(t) * b;
+            const double bd = static_cast<double>(b);
             rc = lazy_init(c, c, n - c, b, s);   // block column: diagonal block + L panel
-            if (rc == HPLAI_OK) rc = launch_getrf(b, a(c, c), lda, B->info + c / b, s);
             const int64_t mrow = n - c - b;
+            if (rc == HPLAI_OK)
+                rc = timed(HPLAI_PHASE_DIAG_LU, 2.0 / 3.0 * bd * bd * bd, s, [&] {
+                    int r = launch_getrf(b, a(c, c), lda, B->info + c / b, s);
+                    if (r == HPLAI_OK && mrow > 0 && b <= 512)
+                        r = launch_diag_inv(b, a(c, c), lda, const_cast<float *>(dinv(c)), s);
+                    return r;
+                });
             if (rc || mrow <= 0) continue;
-            if (b <= 512) rc = launch_diag_inv(b, a(c, c), lda, const_cast<float *>(dinv(c)), s);
+            rc = timed(HPLAI_PHASE_TRSM_L, static_cast<double>(mrow) * bd * bd, s, [&] {
+                return launch_trsm_l(mrow, b, a(c, c), lda, a(c + b, c), lda, aml(buf, t, c + b), b,
+                                     b <= 512 ? dinv(c) : nullptr, s);
+            });
             if (rc == HPLAI_OK)
-                rc = launch_trsm_l(mrow, b, a(c, c), lda, a(c + b, c), lda, aml(buf, t, c + b), b,
-                                   b <= 512 ? dinv(c) : nullptr, s);
-            if (rc == HPLAI_OK)
-                rc = launch_quantize(fmt, 0, mrow, b, b, a(c + b, c), lda, lq(buf, c + b, t), ldq,
-                                     aml(buf, t, c + b), el(buf, t, c + b), s);
+                rc = timed(HPLAI_PHASE_CONV_L, static_cast<double>(mrow) * bd, s, [&] {
+                    return launch_quantize(fmt, 0, mrow, b, b, a(c + b, c), lda, lq(buf, c + b, t), ldq,
+                                           aml(buf, t, c + b), el(buf, t, c + b), s);
+                });
             const int64_t ncols = c0 + Bw - (c + b);
             if (rc || ncols <= 0) continue;
             rc = lazy_init(c, c + b, b, ncols, s);   // block row inside the panel
             if (rc == HPLAI_OK)
-                rc = launch_trsm_u(b, ncols, a(c, c), lda, a(c, c + b), lda, amu(buf, t, c + b), b,
-                                   b <= 512 ? dinv(c) : nullptr, s);
+                rc = timed(HPLAI_PHASE_TRSM_R, static_cast<double>(ncols) * bd * bd, s, [&] {
+                    return launch_trsm_u(b, ncols, a(c, c), lda, a(c, c + b), lda, amu(buf, t, c + b), b,
+                                         b <= 512 ? dinv(c) : nullptr, s);
+                });
             if (rc == HPLAI_OK)
-                rc = launch_quantize(fmt, 1, ncols, b, b, a(c, c + b), lda, uq(buf, c + b, t), ldq,
-                                     amu(buf, t, c + b), eu(buf, t, c + b), s);
+                rc = timed(HPLAI_PHASE_CONV_R, static_cast<double>(ncols) * bd, s, [&] {
+                    return launch_quantize(fmt, 1, ncols, b, b, a(c, c + b), lda, uq(buf, c + b, t), ldq,
+                                           amu(buf, t, c + b), eu(buf, t, c + b), s);
+                });
             if (rc == HPLAI_OK) rc = gemm(buf, t, 1, c + b, c + b, mrow, ncols, s);
         }
         return rc;
@@ -252,13 +364,18 @@ struct LU {
         if (ncols <= 0) return rc;
         for (int t = 0; t < sw && rc == HPLAI_OK; ++t) {
             const int64_t c = c0 + static_cast<int64_t>(t) * b;
+            const double bd = static_cast<double>(b);
             rc = lazy_init(c, col0, b, ncols, s);   // block row right of the outer panel
             if (rc == HPLAI_OK)
-                rc = launch_trsm_u(b, ncols, a(c, c), lda, a(c, col0), lda, amu(buf, t, col0), b,
-                                   b <= 512 ? dinv(c) : nullptr, s);
+                rc = timed(HPLAI_PHASE_TRSM_R, static_cast<double>(ncols) * bd * bd, s, [&] {
+                    return launch_trsm_u(b, ncols, a(c, c), lda, a(c, col0), lda, amu(buf, t, col0), b,
+                                         b <= 512 ? dinv(c) : nullptr, s);
+                });
             if (rc == HPLAI_OK)
-                rc = launch_quantize(fmt, 1, ncols, b, b, a(c, col0), lda, uq(buf, col0, t), ldq,
-                                     amu(buf, t, col0), eu(buf, t, col0), s);
+                rc = timed(HPLAI_PHASE_CONV_R, static_cast<double>(ncols) * bd, s, [&] {
+                    return launch_quantize(fmt, 1, ncols, b, b, a(c, col0), lda, uq(buf, col0, t), ldq,
+                                           amu(buf, t, col0), eu(buf, t, col0), s);
+                });
             if (rc == HPLAI_OK && t + 1 < sw) rc = gemm(buf, t, 1, c + b, col0, col0 - (c + b), ncols, s);
         }
         return rc;
@@ -271,9 +388,14 @@ struct LU {
         int64_t c0 = 0;
         int step = 0;
         int sw = static_cast<int>(std::min<int64_t>(outer, (n - c0) / b));
+        if (epochs) epochs->start(main);
         rc = panel(c0, sw, 0, main);
         while (rc == HPLAI_OK && c0 < n) {
             const int buf = step & 1;
+            if (epochs) {
+                epochs->mark(main, c0);
+                epochs->flush(false);
+            }
             const int64_t Bw = static_cast<int64_t>(sw) * b;
             rc = block_row(c0, sw, buf, main);
             if (rc) break;
@@ -488,6 +610,11 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     GemmTimer timer;
     timer.on = p->profile != 0;
     init_lu(lu, ctx, n, (n + 63) / 64 * 64, b, fmt, outer, p->lazy != 0, p->seed, &B, &timer);
+    EpochLog epochs;
+    epochs.on = p->verbose != 0;
+    epochs.n = n;
+    epochs.b = b;
+    lu.epochs = &epochs;
     hipStream_t s = nullptr;
 
 #define ALLOC(ptr, bytes)                                                             \
@@ -532,6 +659,7 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     // ---- LU ----
     if (rc == HPLAI_OK) rc = lu.run(p->lookahead != 0);
     HPLAI_HIP(hipDeviceSynchronize());
+    epochs.flush(true);
     double t2 = now_s();
     res->time_lu = t2 - t1;
     if (rc == HPLAI_OK) rc = timer.rc;
@@ -551,7 +679,10 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
         for (int it = 0; it < maxit && rc == HPLAI_OK; ++it) {
             HPLAI_HIP(hipMemcpyAsync(B.r, B.rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
             rc = launch_norm_inf(n, B.x, B.norms + 2, s);
-            if (rc == HPLAI_OK) rc = launch_otf_gemv(n, p->seed, B.d, -1.0, B.x, B.r, B.work, s);
+            const double dn2 = static_cast<double>(n) * static_cast<double>(n);
+            if (rc == HPLAI_OK)
+                rc = lu.timed(HPLAI_PHASE_IR_GEMV, 2.0 * dn2, s,
+                              [&] { return launch_otf_gemv(n, p->seed, B.d, -1.0, B.x, B.r, B.work, s); });
             if (rc == HPLAI_OK) rc = launch_norm_inf(n, B.r, B.norms + 3, s);
             if (rc) break;
             HPLAI_HIP(hipMemcpy(hn + 2, B.norms + 2, sizeof(double) * 2, hipMemcpyDeviceToHost));
@@ -570,8 +701,11 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
                 converged = true;
                 break;
             }
-            rc = launch_trsv_lower(n, b, B.A, lu.lda, B.r, s);
-            if (rc == HPLAI_OK) rc = launch_trsv_upper(n, b, B.A, lu.lda, B.r, s);
+            rc = lu.timed(HPLAI_PHASE_IR_TRSV, 2.0 * dn2, s, [&] {
+                int r = launch_trsv_lower(n, b, B.A, lu.lda, B.r, s);
+                if (r == HPLAI_OK) r = launch_trsv_upper(n, b, B.A, lu.lda, B.r, s);
+                return r;
+            });
             if (rc == HPLAI_OK) rc = launch_vec_add(n, B.r, B.x, s);
         }
     }
@@ -579,6 +713,13 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     double t3 = now_s();
     res->time_ir = t3 - t2;
     res->time_total = t3 - t1;
+    if (p->profile) {
+        timer.collect(res);   // the IR phases recorded after the LU
+        res->phase_seconds[HPLAI_PHASE_TOTAL] = res->time_total;
+        double rest = res->time_total;
+        for (int i = 0; i < HPLAI_PHASE_TOTAL; ++i) rest -= res->phase_seconds[i];
+        res->phase_seconds[HPLAI_PHASE_MISC] = rest;
+    }
     res->ir_steps = steps;
     const double dn = static_cast<double>(n);
     res->gflops = (2. / 3 * dn * dn * dn + 3. / 2 * dn * dn) / res->time_total * 1e-9;

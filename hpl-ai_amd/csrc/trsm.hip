@@ -222,6 +222,188 @@ __global__ __launch_bounds__(256) void trsm_l_strip_kernel(int64_t m, int b, con
     tile_max_reduce(amax, i0 / tb, vmax, true);
 }
 
+// ---------------------------------------------------------------------------
+// MFMA forms (b % 32 == 0): the same strip algorithm with every 32 x 32 x 32
+// product on v_mfma_f32_32x32x2_f32 (FP32 inputs, FP32 accumulation), twice
+// the FP32 rate of the vector ALU.  Operands: the strip from LDS, the panel
+// block from global memory (L2-resident), each lane loading its 128-byte
+// column segment once and picking the even or odd K entries it feeds.
+// Output layout of the MFMA: acc[4 q + r] is row 8 q + 4 (lane / 32) + r,
+// column lane % 32 of the 32 x 32 block.
+// ---------------------------------------------------------------------------
+typedef float floatx16t_t __attribute__((ext_vector_type(16)));
+
+// v[0..15] = seg[2 kk + h] for kk = 0..15 from a 32-float contiguous segment
+__device__ __forceinline__ void load_k_pairs(const float *__restrict__ seg, int h, float (&v)[16])
+{
+    float t[32];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const float4 q = reinterpret_cast<const float4 *>(seg)[e];
+        t[4 * e] = q.x;
+        t[4 * e + 1] = q.y;
+        t[4 * e + 2] = q.z;
+        t[4 * e + 3] = q.w;
+    }
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) v[kk] = h ? t[2 * kk + 1] : t[2 * kk];
+}
+
+// L(m x b) := L Ukk^-1 (update_left_panels, panel_trf.hpp:143-157), b % 32 == 0.
+__global__ __launch_bounds__(256) void trsm_l_mfma_kernel(int64_t m, int b, const float *__restrict__ Uk,
+                                                          int64_t ldu, const float *__restrict__ inv,
+                                                          float *__restrict__ L, int64_t ldl,
+                                                          float *__restrict__ amax, int tb)
+{
+    __shared__ float X[kPanelMax * kXS];   // X[c * kXS + r]: column c, row i0 + r
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, lh = lane >> 5;
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kStrip;
+    const int nr = m - i0 < kStrip ? static_cast<int>(m - i0) : kStrip;
+    const int tot = b * kStrip;
+    for (int base = 0; base < tot; base += 256 * 16) {
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = min(base + u * 256 + tid, tot - 1);
+            const int c = idx / kStrip, r = idx % kStrip;
+            v[u] = L[static_cast<int64_t>(c) * ldl + i0 + min(r, nr - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = base + u * 256 + tid;
+            const int c = idx / kStrip, r = idx % kStrip;
+            if (idx < tot) X[c * kXS + r] = r < nr ? v[u] : 0.0f;
+        }
+    }
+    __syncthreads();
+    for (int jb = 0; jb < b; jb += kStrip) {
+        // X(:, jb:jb+32) := X(:, jb:jb+32) Uinv  (wave 0)
+        if (wave == 0) {
+            const float *Ui = inv + static_cast<int64_t>(jb / kStrip) * kInvBlock + 1024;   // row-major
+            floatx16t_t acc;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk) {
+                const int k = 2 * kk + lh;
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(X[(jb + k) * kXS + l32], Ui[k * 32 + l32], acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) X[(jb + l32) * kXS + 8 * q + 4 * lh + r] = acc[4 * q + r];
+        }
+        __syncthreads();
+        // X(:, c0:c0+32) -= X(:, jb:jb+32) U(jb:jb+32, c0:c0+32) for the blocks right of jb
+        const int nblk = (b - jb - kStrip) / kStrip;
+        for (int blk = wave; blk < nblk; blk += 4) {
+            const int c0 = jb + kStrip + blk * kStrip;
+            float bv[16];
+            load_k_pairs(Uk + jb + static_cast<int64_t>(c0 + l32) * ldu, lh, bv);
+            floatx16t_t acc;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(X[(jb + 2 * kk + lh) * kXS + l32], bv[kk], acc, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) X[(c0 + l32) * kXS + 8 * q + 4 * lh + r] -= acc[4 * q + r];
+        }
+        __syncthreads();
+    }
+    float vmax = 0.0f;
+    for (int idx = tid; idx < b * kStrip; idx += 256) {
+        const int c = idx / kStrip, r = idx % kStrip;
+        if (r < nr) {
+            const float v = X[c * kXS + r];
+            L[static_cast<int64_t>(c) * ldl + i0 + r] = v;
+            vmax = fmaxf(vmax, fabsf(v));
+        }
+    }
+    tile_max_reduce(amax, i0 / tb, vmax, true);
+}
+
+// U(b x n) := Lkk^-1 U (update_right_panels, panel_trf.hpp:158-171), b % 32 == 0.
+__global__ __launch_bounds__(256) void trsm_u_mfma_kernel(int b, int64_t n, const float *__restrict__ Lk,
+                                                          int64_t ldl, const float *__restrict__ inv,
+                                                          float *__restrict__ U, int64_t ldu,
+                                                          float *__restrict__ amax, int tb)
+{
+    __shared__ float X[kPanelMax * kXS];   // X[r * kXS + c]: row r of column j0 + c
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, lh = lane >> 5;
+    const int64_t j0 = static_cast<int64_t>(blockIdx.x) * kStrip;
+    const int nc = n - j0 < kStrip ? static_cast<int>(n - j0) : kStrip;
+    const int tot = b * kStrip;
+    for (int base = 0; base < tot; base += 256 * 16) {
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = min(base + u * 256 + tid, tot - 1);
+            const int c = idx / b, r = idx % b;
+            v[u] = U[(j0 + min(c, nc - 1)) * ldu + r];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = base + u * 256 + tid;
+            const int c = idx / b, r = idx % b;
+            if (idx < tot) X[r * kXS + c] = c < nc ? v[u] : 0.0f;
+        }
+    }
+    __syncthreads();
+    for (int kb = 0; kb < b; kb += kStrip) {
+        // X(kb:kb+32, :) := Linv X(kb:kb+32, :)  (wave 0)
+        if (wave == 0) {
+            const float *Li = inv + static_cast<int64_t>(kb / kStrip) * kInvBlock;   // row-major Linv
+            float av[16];
+            load_k_pairs(Li + l32 * 32, lh, av);
+            floatx16t_t acc;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], X[(kb + 2 * kk + lh) * kXS + l32], acc, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) X[(kb + 8 * q + 4 * lh + r) * kXS + l32] = acc[4 * q + r];
+        }
+        __syncthreads();
+        // X(r0:r0+32, :) -= L(r0:r0+32, kb:kb+32) X(kb:kb+32, :) for the blocks below kb
+        const int nblk = (b - kb - kStrip) / kStrip;
+        for (int blk = wave; blk < nblk; blk += 4) {
+            const int r0 = kb + kStrip + blk * kStrip;
+            float av[16];
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk) av[kk] = Lk[(r0 + l32) + static_cast<int64_t>(kb + 2 * kk + lh) * ldl];
+            floatx16t_t acc;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], X[(kb + 2 * kk + lh) * kXS + l32], acc, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) X[(r0 + 8 * q + 4 * lh + r) * kXS + l32] -= acc[4 * q + r];
+        }
+        __syncthreads();
+    }
+    float vmax = 0.0f;
+    for (int idx = tid; idx < b * kStrip; idx += 256) {
+        const int c = idx / b, r = idx % b;
+        if (c < nc) {
+            const float v = X[r * kXS + c];
+            U[(j0 + c) * ldu + r] = v;
+            vmax = fmaxf(vmax, fabsf(v));
+        }
+    }
+    tile_max_reduce(amax, j0 / tb, vmax, true);
+}
+
 // The strip kernels need b <= 512 and, when tile maxima are collected, every
 // strip inside one tile (tb % 32 == 0).
 bool trsm_strip_ok(int b, const float *amax, int tb)
@@ -242,8 +424,12 @@ int launch_diag_inv(int b, const float *LU, int64_t lda, float *inv, hipStream_t
 int launch_trsm_l_strip(int64_t m, int b, const float *U, int64_t ldu, const float *inv, float *L,
                         int64_t ldl, float *amax, int tb, hipStream_t s)
 {
-    hipLaunchKernelGGL(trsm_l_strip_kernel, dim3(static_cast<unsigned>(ceil_div(m, kStrip))), dim3(256),
-                       0, s, m, b, U, ldu, inv, L, ldl, amax, tb);
+    if (b % kStrip == 0 && (ldu % 4) == 0 && (reinterpret_cast<uintptr_t>(U) % 16) == 0)
+        hipLaunchKernelGGL(trsm_l_mfma_kernel, dim3(static_cast<unsigned>(ceil_div(m, kStrip))), dim3(256),
+                           0, s, m, b, U, ldu, inv, L, ldl, amax, tb);
+    else
+        hipLaunchKernelGGL(trsm_l_strip_kernel, dim3(static_cast<unsigned>(ceil_div(m, kStrip))), dim3(256),
+                           0, s, m, b, U, ldu, inv, L, ldl, amax, tb);
     HPLAI_HIP(hipGetLastError());
     return HPLAI_OK;
 }
@@ -251,8 +437,12 @@ int launch_trsm_l_strip(int64_t m, int b, const float *U, int64_t ldu, const flo
 int launch_trsm_u_strip(int b, int64_t n, const float *Lkk, int64_t ldl, const float *inv, float *U,
                         int64_t ldu, float *amax, int tb, hipStream_t s)
 {
-    hipLaunchKernelGGL(trsm_u_strip_kernel, dim3(static_cast<unsigned>(ceil_div(n, kStrip))), dim3(256),
-                       0, s, b, n, Lkk, ldl, inv, U, ldu, amax, tb);
+    if (b % kStrip == 0)
+        hipLaunchKernelGGL(trsm_u_mfma_kernel, dim3(static_cast<unsigned>(ceil_div(n, kStrip))), dim3(256),
+                           0, s, b, n, Lkk, ldl, inv, U, ldu, amax, tb);
+    else
+        hipLaunchKernelGGL(trsm_u_strip_kernel, dim3(static_cast<unsigned>(ceil_div(n, kStrip))), dim3(256),
+                           0, s, b, n, Lkk, ldl, inv, U, ldu, amax, tb);
     HPLAI_HIP(hipGetLastError());
     return HPLAI_OK;
 }

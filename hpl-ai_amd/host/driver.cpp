@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <unistd.h>
 #include <ctime>
 
 #include "hplai.h"
@@ -35,6 +36,8 @@ static void usage()
         "   --lookahead <0|1> Factor panel k+1 on a second stream (default 1).\n"
         "   --seed <s>        Generator seed (default 42).\n"
         "   --device <d>      HIP device (default 0).\n"
+        "   --job-id <id>     Job id in the Timerdump file name (default 0).\n"
+        "   --no-timerdump    Do not time the phases or write Timerdump.<jobid>.<n>.<b>.1.0.\n"
         "   --help, -h        Print this message.\n");
 }
 
@@ -43,6 +46,36 @@ static void print_ctime(const char *msg)
     std::time_t t;
     std::time(&t);
     std::printf("%s%s", msg, std::ctime(&t));
+}
+
+// Timerdump file of the reference (main.cpp:470-478, Timer::show / dump_mp
+// timer.hpp:167-214): one line per phase, then the Schur GEMM rate.
+static void write_timerdump(bool warmup, int jobid, long long n, int b, const hplai_result &r)
+{
+    static const char *names[HPLAI_PHASES] = {
+        "DIAG_BCAST", "LCOL_BCAST", "RROW_BCAST", "TEST", "WAIT", "DIAG_LU", "TRSM_L", "TRSM_R",
+        "CONV_L", "CONV_R", "GEMM_UPDATE", "GEMM_PROGRESS", "LAZY_INIT", "WRITE_BACK", "IR_GEMV",
+        "IR_GEMV_COMM", "IR_TRSV", "IR_TRSV_MV", "IR_TRSV_COMM", "TOTAL", "MISC"};
+    char filename[1024];
+    std::snprintf(filename, sizeof(filename),
+                  warmup ? "Warmup.%06d.%09lld.%04d.%05d.%05d" : "Timerdump.%06d.%09lld.%04d.%05d.%05d", jobid,
+                  n, b, 1, 0);
+    FILE *fp = std::fopen(filename, "w");
+    if (!fp) return;
+    char host[256] = "unknown";
+    (void)gethostname(host, sizeof(host) - 1);
+    std::fprintf(fp, "# row=%d, col=%d, host=%s\n", 0, 0, host);
+    const double total = r.phase_seconds[HPLAI_PHASE_TOTAL];
+    for (int i = 0; i < HPLAI_PHASES; ++i) {
+        const double t = r.phase_seconds[i];
+        std::fprintf(fp, " %-12s : %e sec : %6.2f %% : %20lld : %e Gop/s\n", names[i], t,
+                     total > 0 ? 100.0 * t / total : 0.0, (long long)r.phase_ops[i],
+                     t > 0 ? 1e-9 * r.phase_ops[i] / t : 0.0);
+    }
+    const double gt = r.phase_seconds[HPLAI_PHASE_GEMM_UPDATE] + r.phase_seconds[HPLAI_PHASE_GEMM_PROGRESS];
+    const double gf = r.phase_ops[HPLAI_PHASE_GEMM_UPDATE] + r.phase_ops[HPLAI_PHASE_GEMM_PROGRESS];
+    std::fprintf(fp, "GEMM_TOTAL: %f Tflops\n", gt > 0 ? 1.e-12 * gf / gt : 0.0);
+    std::fclose(fp);
 }
 
 int main(int argc, char **argv)
@@ -62,6 +95,8 @@ int main(int argc, char **argv)
     p.lookahead = 1;
     p.maxit = 50;
     p.verbose = 1;
+    p.profile = 1;   // per-phase times for the Timerdump file
+    int jobid = 0;
     bool checksum = false, warmup = false, lazy = false, rdma = false, tile = true;
     int device = 0;
     for (int i = 4; i < argc; ++i) {
@@ -84,12 +119,15 @@ int main(int argc, char **argv)
         else if (!std::strcmp(a, "--lookahead")) p.lookahead = std::atoi(next());
         else if (!std::strcmp(a, "--seed")) p.seed = std::strtoull(next(), nullptr, 10);
         else if (!std::strcmp(a, "--device")) device = std::atoi(next());
+        else if (!std::strcmp(a, "--job-id")) jobid = std::atoi(next());
+        else if (!std::strcmp(a, "--no-timerdump")) p.profile = 0;
         else if (!std::strcmp(a, "--help") || !std::strcmp(a, "-h")) { usage(); return 0; }
         else std::printf("Ignoring unknown option: %s\n", a);
     }
     if (warmup) p.maxit = 1;
     p.lazy = lazy ? 1 : 0;
 
+    std::printf("jobid=%d\n", jobid);
     std::printf("n=%lld b=%d r=%d c=%d\n", n, b, 1, 1);
     std::printf("%s %s %s %s nopack %s noskiplu ddmat\n", tile ? "tile" : "2dbc",
                 lazy ? "lazy" : "precomp", rdma ? "rdma" : "mpi", warmup ? "warmup" : "full",
@@ -97,6 +135,7 @@ int main(int argc, char **argv)
     std::printf("device=%d P=%d fmt=%s outer=%d lookahead=%d\n", device, nrow,
                 p.fmt == HPLAI_FMT_F16 ? "f16" : (p.fmt == HPLAI_FMT_F16_TILE ? "f16t" : "e4m3"),
                 p.outer, p.lookahead);
+    std::printf("epoch_size = %d\n", b);   // b * lcm(1, 1) on a 1 x 1 grid (main.cpp:193)
     print_ctime("#BEGIN: ");
     std::fflush(stdout);
 
@@ -120,6 +159,7 @@ int main(int argc, char **argv)
     std::printf("# generation %.6f s, LU %.6f s, IR %.6f s (%d residual evaluations)\n", r.time_gen,
                 r.time_lu, r.time_ir, r.ir_steps);
     std::fflush(stdout);
+    if (p.profile) write_timerdump(warmup, jobid, n, b, r);
     hplai_destroy(ctx);
     return rc == HPLAI_OK ? 0 : 3;
 }

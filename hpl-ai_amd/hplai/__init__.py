@@ -40,7 +40,11 @@ class Result(ctypes.Structure):
                 ("schur_launches", _i64), ("schur_flops", _dbl), ("schur_ms", _dbl),
                 ("schur_large_launches", _i64), ("schur_large_flops", _dbl), ("schur_large_ms", _dbl),
                 ("schur_large_bytes", _dbl), ("schur_inner_launches", _i64), ("schur_inner_flops", _dbl),
-                ("schur_inner_ms", _dbl)]
+                ("schur_inner_ms", _dbl), ("phase_seconds", _dbl * 21), ("phase_ops", _dbl * 21)]
+
+PHASES = ["DIAG_BCAST", "LCOL_BCAST", "RROW_BCAST", "TEST", "WAIT", "DIAG_LU", "TRSM_L", "TRSM_R", "CONV_L",
+          "CONV_R", "GEMM_UPDATE", "GEMM_PROGRESS", "LAZY_INIT", "WRITE_BACK", "IR_GEMV", "IR_GEMV_COMM",
+          "IR_TRSV", "IR_TRSV_MV", "IR_TRSV_COMM", "TOTAL", "MISC"]
 
 
 # (name, restype, argtypes) for every exported entry point of include/hplai.h
@@ -242,7 +246,8 @@ class Context:
                     schur_large_launches=r.schur_large_launches, schur_large_flops=r.schur_large_flops,
                     schur_large_ms=r.schur_large_ms, schur_large_bytes=r.schur_large_bytes,
                     schur_inner_launches=r.schur_inner_launches, schur_inner_flops=r.schur_inner_flops,
-                    schur_inner_ms=r.schur_inner_ms)
+                    schur_inner_ms=r.schur_inner_ms,
+                    phases={PHASES[i]: (r.phase_seconds[i], r.phase_ops[i]) for i in range(21)})
 
     def solution_x(self, n):
         import numpy as np

@@ -191,10 +191,35 @@ typedef struct hplai_params {
                             matrix starts at zero and accumulates -sum L U; each block gets
                             its generated a_ij added when it joins a panel                */
     int      maxit;      /* IR iterations (50 in the reference, 1 under -w)        */
-    int      verbose;    /* print the reference's IR trace lines                   */
+    int      verbose;    /* print the reference's IR trace and !epoch lines        */
     int      keep_factors; /* leave A (factors) allocated for hplai_solution_*     */
     int      profile;    /* time every Schur-update launch with HIP events         */
 } hplai_params;
+
+/* Phases of the reference's Timer (timer.hpp:58-83), same order and names;
+ * the communication phases are 0 on one GPU. */
+#define HPLAI_PHASE_DIAG_BCAST    0
+#define HPLAI_PHASE_LCOL_BCAST    1
+#define HPLAI_PHASE_RROW_BCAST    2
+#define HPLAI_PHASE_TEST          3
+#define HPLAI_PHASE_WAIT          4
+#define HPLAI_PHASE_DIAG_LU       5
+#define HPLAI_PHASE_TRSM_L        6
+#define HPLAI_PHASE_TRSM_R        7
+#define HPLAI_PHASE_CONV_L        8
+#define HPLAI_PHASE_CONV_R        9
+#define HPLAI_PHASE_GEMM_UPDATE   10
+#define HPLAI_PHASE_GEMM_PROGRESS 11
+#define HPLAI_PHASE_LAZY_INIT     12
+#define HPLAI_PHASE_WRITE_BACK    13
+#define HPLAI_PHASE_IR_GEMV       14
+#define HPLAI_PHASE_IR_GEMV_COMM  15
+#define HPLAI_PHASE_IR_TRSV       16
+#define HPLAI_PHASE_IR_TRSV_MV    17
+#define HPLAI_PHASE_IR_TRSV_COMM  18
+#define HPLAI_PHASE_TOTAL         19
+#define HPLAI_PHASE_MISC          20
+#define HPLAI_PHASES              21
 
 typedef struct hplai_result {
     double time_gen;     /* matrix + diag + rhs generation (not in time_total)     */
@@ -223,6 +248,13 @@ typedef struct hplai_result {
     int64_t schur_inner_launches;
     double  schur_inner_flops;
     double  schur_inner_ms;
+    /* with profile: per-phase device time (seconds, summed kernel durations
+       from HIP events; phases on the two streams overlap) and operation
+       counts as the reference's Timer accumulates them; TOTAL is the LU + IR
+       wall time and MISC = TOTAL - the other phases.  IR_TRSV covers the
+       whole triangular solves (diagonal blocks and panel products). */
+    double  phase_seconds[HPLAI_PHASES];
+    double  phase_ops[HPLAI_PHASES];
 } hplai_result;
 
 /* Generate, factor and refine; synchronous.  Returns HPLAI_ENOCONV if the

@@ -54,7 +54,7 @@ def _c_layout(tmp_path):
 int main(void) {
   printf("%zu %zu %zu %zu %zu\n", sizeof(hplai_params), offsetof(hplai_params, profile),
          sizeof(hplai_result), offsetof(hplai_result, schur_ms),
-         offsetof(hplai_result, schur_inner_ms));
+         offsetof(hplai_result, phase_ops));
   return 0;
 }''')
     exe = tmp_path / "layout"
@@ -70,7 +70,7 @@ def test_struct_layouts_match_c(tmp_path):
     assert hplai.Params.profile.offset == op
     assert ctypes.sizeof(hplai.Result) == sr
     assert hplai.Result.schur_ms.offset == orr
-    assert hplai.Result.schur_inner_ms.offset == orl
+    assert hplai.Result.phase_ops.offset == orl
 
 
 def test_driver_help():

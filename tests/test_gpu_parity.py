@@ -110,7 +110,7 @@ def test_getrf_zero_pivot_reported(ctx):
     assert int(info.cpu()[0]) == 6
 
 
-@pytest.mark.parametrize("m,b", [(1140, 60), (3584, 512), (777, 100)])
+@pytest.mark.parametrize("m,b", [(1140, 60), (3584, 512), (777, 100), (1000, 128), (40000, 512)])
 def test_trsm_lpanel(ctx, oracle, m, b):
     n = m + b
     D = _diag_block(oracle, n, b, 0)
@@ -125,7 +125,7 @@ def test_trsm_lpanel(ctx, oracle, m, b):
     assert rel < 2e-5, rel
 
 
-@pytest.mark.parametrize("b,ncol", [(60, 1140), (512, 3584), (100, 777)])
+@pytest.mark.parametrize("b,ncol", [(60, 1140), (512, 3584), (100, 777), (128, 1000), (512, 40000)])
 def test_trsm_upanel(ctx, oracle, b, ncol):
     n = ncol + b
     D = _diag_block(oracle, n, b, 0)
@@ -327,3 +327,31 @@ def test_solve_b512(ctx, fmt, outer, lookahead, lazy):
     res = ctx.solve(n, b, fmt=fmt, outer=outer, lookahead=lookahead, lazy=lazy)
     assert res["converged"] and res["harness"] < 16.0, res
     assert res["ir_steps"] <= 10
+
+
+# ------------------------------------------------------------------ driver.out
+def test_driver_output_and_timerdump(tmp_path):
+    """driver.out at the reference's C1 configuration: the reference's output
+    lines (IR trace, !epoch progress, result line) and its Timerdump file."""
+    import os
+    import re
+    import subprocess
+    import hplai
+    if not os.path.exists(hplai.DRIVER_PATH):
+        pytest.skip("driver.out not built")
+    out = subprocess.run([hplai.DRIVER_PATH, "1200", "60", "2", "-not", "--checksum", "--job-id", "7"],
+                         capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr
+    text = out.stdout
+    assert "jobid=7" in text and "epoch_size = 60" in text
+    assert re.search(r"^!epoch 0/20: elapsed=[0-9.]+, [0-9.]+ Pflops \(estimate\)$", text, re.M)
+    assert re.search(r"^# iterative refinement: step=  0, residual=3\.366149436393\d+e-02 hpl-harness=", text, re.M)
+    assert re.search(r"^[0-9.]+ sec\. [0-9.]+ GFlop/s resid = [0-9.e+-]+ hpl-harness = [0-9.]+$", text, re.M)
+    dump = tmp_path / "Timerdump.000007.000001200.0060.00001.00000"
+    lines = dump.read_text().splitlines()
+    assert lines[0].startswith("# row=0, col=0, host=")
+    names = [ln.split(":")[0].strip() for ln in lines[1:22]]
+    assert names == hplai.PHASES
+    assert lines[22].startswith("GEMM_TOTAL: ")
+    sec = {ln.split(":")[0].strip(): float(ln.split(":")[1].split()[0]) for ln in lines[1:22]}
+    assert sec["TOTAL"] > 0 and sec["DIAG_LU"] > 0 and sec["GEMM_UPDATE"] > 0 and sec["IR_TRSV"] > 0
