@@ -242,9 +242,9 @@ def main():
     ap.add_argument("--n", type=int, default=196608)
     ap.add_argument("--b", type=int, default=512)
     ap.add_argument("--fmt", default="e4m3", choices=["e4m3", "f16", "f16t"])
-    ap.add_argument("--outer", type=int, default=8)
+    ap.add_argument("--outer", type=int, default=16)
     ap.add_argument("--compare", default="f16", help="second form run beside the primary ('' = none)")
-    ap.add_argument("--compare-outer", type=int, default=8)
+    ap.add_argument("--compare-outer", type=int, default=16)
     ap.add_argument("--no-lookahead", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--cpu-b", type=int, default=512)

@@ -12,12 +12,16 @@
 //    step with K = B, which divides its HBM traffic by `outer`.  With outer = 1
 //    this is exactly the reference's right-looking schedule.
 // Per outer step (columns c0 .. c0+B):
-//   1. panel: for each inner block t: getrf, L := A U^-1 (rows below),
-//      quantize L; U := L^-1 A inside the panel, quantize, and update the rest
-//      of the panel (K = b);
-//   2. block row: for each t: U12_t := L_tt^-1 A12_t, quantize, update the
-//      block rows below it inside the block row (K = b);
+//   1. panel: for each inner block t: the rows below the outer block of
+//      block column t take the earlier inner blocks at once (left-looking,
+//      K = t b); getrf, L := A U^-1 (rows below), quantize L; U := L^-1 A for
+//      the rest of block row t inside the outer block, quantize, and update
+//      the square part of the outer block right of and below t (K = b);
+//   2. block row: for each t: row block t takes the earlier row blocks at
+//      once (left-looking, K = t b), then U12_t := L_tt^-1 A12_t, quantize;
 //   3. trailing: A22 -= Lq(:, 0..B) Uq(0..B, :) (K = B).
+// The left-looking parts write each block column / row block of the outer
+// block once instead of once per inner block, which cuts their HBM traffic.
 // With lookahead, step 3 first updates the next outer panel's columns, then
 // the panel factorization of step k+1 runs on a second stream while the rest
 // of step 3 runs on the main stream.
@@ -318,7 +322,11 @@ struct LU {
         for (int t = 0; t < sw && rc == HPLAI_OK; ++t) {
             const int64_t c = c0 + static_cast<int64_t>(t) * b;
             const double bd = static_cast<double>(b);
-            rc = lazy_init(c, c, n - c, b, s);   // block column: diagonal block + L panel
+            // rows below the outer block take all earlier panels of the block
+            // at once (left-looking, K = t b); the rows inside the block were
+            // updated panel by panel below
+            if (t > 0 && n - (c0 + Bw) > 0) rc = gemm(buf, 0, t, c0 + Bw, c, n - (c0 + Bw), b, s);
+            if (rc == HPLAI_OK) rc = lazy_init(c, c, n - c, b, s);   // block column: diagonal block + L panel
             const int64_t mrow = n - c - b;
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_DIAG_LU, 2.0 / 3.0 * bd * bd * bd, s, [&] {
@@ -350,7 +358,9 @@ struct LU {
                     return launch_quantize(fmt, 1, ncols, b, b, a(c, c + b), lda, uq(buf, c + b, t), ldq,
                                            amu(buf, t, c + b), eu(buf, t, c + b), s);
                 });
-            if (rc == HPLAI_OK) rc = gemm(buf, t, 1, c + b, c + b, mrow, ncols, s);
+            // right-looking inside the outer block only (rows and columns
+            // c + b .. c0 + Bw, K = b)
+            if (rc == HPLAI_OK) rc = gemm(buf, t, 1, c + b, c + b, ncols, ncols, s);
         }
         return rc;
     }
@@ -365,7 +375,9 @@ struct LU {
         for (int t = 0; t < sw && rc == HPLAI_OK; ++t) {
             const int64_t c = c0 + static_cast<int64_t>(t) * b;
             const double bd = static_cast<double>(b);
-            rc = lazy_init(c, col0, b, ncols, s);   // block row right of the outer panel
+            // row block t takes the earlier row blocks at once (left-looking, K = t b)
+            if (t > 0) rc = gemm(buf, 0, t, c, col0, b, ncols, s);
+            if (rc == HPLAI_OK) rc = lazy_init(c, col0, b, ncols, s);   // block row right of the outer panel
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_TRSM_R, static_cast<double>(ncols) * bd * bd, s, [&] {
                     return launch_trsm_u(b, ncols, a(c, c), lda, a(c, col0), lda, amu(buf, t, col0), b,
@@ -376,7 +388,6 @@ struct LU {
                     return launch_quantize(fmt, 1, ncols, b, b, a(c, col0), lda, uq(buf, col0, t), ldq,
                                            amu(buf, t, col0), eu(buf, t, col0), s);
                 });
-            if (rc == HPLAI_OK && t + 1 < sw) rc = gemm(buf, t, 1, c + b, col0, col0 - (c + b), ncols, s);
         }
         return rc;
     }

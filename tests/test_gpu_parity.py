@@ -314,6 +314,27 @@ def test_solve_c1_matches_oracle(ctx, oracle):
     assert np.max(np.abs(x - xr)) / np.max(np.abs(xr)) < 1e-10
 
 
+@pytest.mark.parametrize("outer,lookahead,lazy", [(4, True, False), (5, True, True), (20, False, False)])
+def test_solve_outer_blocked_matches_oracle(ctx, oracle, outer, lookahead, lazy):
+    """The outer-blocked schedule (left-looking inside an outer block of
+    `outer` panels, one K = outer*b trailing update) quantizes every L and U
+    panel at the same point of the elimination as the reference's
+    right-looking schedule, so its factors differ from the oracle's only by
+    FP32 summation order: the same 1e-4 bound as the outer = 1 case, and the
+    same converged solution."""
+    n, b = 1200, 60
+    res = ctx.solve(n, b, fmt="f16", outer=outer, lookahead=lookahead, lazy=lazy, keep_factors=True)
+    lu = ctx.solution_factors(n)
+    x = ctx.solution_x(n)
+    ctx.release()
+    ref, d = oracle.lu(oracle.LU_PRECOMP, n, b, SEED)
+    assert res["converged"] and res["harness"] < 16.0 and res["ir_steps"] <= 4
+    rel = np.max(np.abs(lu - ref)) / np.max(np.abs(ref))
+    assert rel < 1e-4, rel
+    xr = oracle.ir(ref, n, d)["x"]
+    assert np.max(np.abs(x - xr)) / np.max(np.abs(xr)) < 1e-10
+
+
 def test_solve_c1_lazy(ctx):
     res = ctx.solve(1200, 60, fmt="f16", lazy=True)
     assert res["converged"] and res["harness"] < 16.0 and res["ir_steps"] <= 4
