@@ -1,12 +1,15 @@
 // trsm.hip — panel TRSMs of the blocked LU with the whole panel height in LDS.
 //
 // One workgroup owns a strip of 32 columns (U panel) or 32 rows (L panel) of
-// the panel and all b <= 512 entries of its other dimension, in LDS.  The
-// strip is solved 32 at a time: the 32 x 32 diagonal sub-block is applied
-// through its precomputed inverse (a dense 32 x 32 product, parallel over the
-// workgroup), then a rank-32 update removes the solved part from the rest of
-// the strip.  The inverses of the diagonal sub-blocks of L and U come from
-// diag_inv32_kernel, run once per factored diagonal block.
+// the panel and all b <= 512 entries of its other dimension, in LDS.
+//  * b % 32 == 0: the triangles of the factored diagonal block are inverted
+//    in full once per block (tri_inv_kernel, after the 32 x 32 sub-block
+//    inverses of diag_inv32_kernel), and every strip is a triangular matrix
+//    product with the inverse on the FP32 MFMA (trsm_l_tri_kernel,
+//    trsm_u_tri_kernel).
+//  * otherwise: the strip is solved 32 at a time: the 32 x 32 diagonal
+//    sub-block through its inverse, then a rank-32 update of the rest of the
+//    strip (trsm_l_strip_kernel, trsm_u_strip_kernel).
 // Panels with b > 512 use the register-blocked kernels in panel.hip.
 #include "device_common.hpp"
 #include "internal.hpp"
@@ -223,13 +226,19 @@ __global__ __launch_bounds__(256) void trsm_l_strip_kernel(int64_t m, int b, con
 }
 
 // ---------------------------------------------------------------------------
-// MFMA forms (b % 32 == 0): the same strip algorithm with every 32 x 32 x 32
-// product on v_mfma_f32_32x32x2_f32 (FP32 inputs, FP32 accumulation), twice
-// the FP32 rate of the vector ALU.  Operands: the strip from LDS, the panel
-// block from global memory (L2-resident), each lane loading its 128-byte
-// column segment once and picking the even or odd K entries it feeds.
-// Output layout of the MFMA: acc[4 q + r] is row 8 q + 4 (lane / 32) + r,
-// column lane % 32 of the 32 x 32 block.
+// MFMA forms (b % 32 == 0, b <= 512).  The diagonal block's triangles are
+// inverted in full once per factored block (tri_inv_kernel); each panel solve
+// is then a triangular matrix product (TRMM) with that inverse,
+//   L := A Uinv,  U := Linv A,
+// whose 32 x 32 output blocks are independent, so the four waves of a
+// workgroup run without barriers between them instead of walking the
+// 32-wide substitution chain.  Products run on v_mfma_f32_32x32x2_f32 (FP32
+// inputs and accumulation).  The strip (32 rows or columns, all b of the
+// other dimension) is in LDS; the inverse is read from global memory
+// (L2-resident) in MFMA operand order, 16 consecutive floats per lane.
+// MFMA layout: A operand lane -> (row lane % 32, k 2 kk + lane / 32), B
+// operand lane -> (k 2 kk + lane / 32, column lane % 32); output acc[4 q + r]
+// is row 8 q + 4 (lane / 32) + r, column lane % 32.
 // ---------------------------------------------------------------------------
 typedef float floatx16t_t __attribute__((ext_vector_type(16)));
 
@@ -249,11 +258,116 @@ __device__ __forceinline__ void load_k_pairs(const float *__restrict__ seg, int 
     for (int kk = 0; kk < 16; ++kk) v[kk] = h ? t[2 * kk + 1] : t[2 * kk];
 }
 
-// L(m x b) := L Ukk^-1 (update_left_panels, panel_trf.hpp:143-157), b % 32 == 0.
-__global__ __launch_bounds__(256) void trsm_l_mfma_kernel(int64_t m, int b, const float *__restrict__ Uk,
-                                                          int64_t ldu, const float *__restrict__ inv,
-                                                          float *__restrict__ L, int64_t ldl,
-                                                          float *__restrict__ amax, int tb)
+// Inverse layout after the nb = b / 32 sub-block inverses (kInvBlock each):
+//   LinvP: block (I, J) at (I nb + J) 1024, [lane 16 + kk] = Linv(32 I + lane % 32, 32 J + 2 kk + lane / 32)
+//          (A-operand order for U := Linv A)
+//   UinvP: block (I, J) at nb^2 1024 + (J nb + I) 1024,
+//          [lane 16 + kk] = Uinv(32 I + 2 kk + lane / 32, 32 J + lane % 32)
+//          (B-operand order for L := A Uinv)
+// Blocks above the diagonal of Linv and below the diagonal of Uinv are never
+// written or read.
+__device__ __forceinline__ void tri_store(float *tri, int nb, bool upper, int I, int J, int rho, int gam, float v)
+{
+    if (upper)
+        tri[static_cast<int64_t>(nb) * nb * 1024 + (static_cast<int64_t>(J) * nb + I) * 1024 +
+            (gam + 32 * (rho & 1)) * 16 + (rho >> 1)] = v;
+    else
+        tri[(static_cast<int64_t>(I) * nb + J) * 1024 + (rho + 32 * (gam & 1)) * 16 + (gam >> 1)] = v;
+}
+
+// One workgroup per 32-column block j of Linv (blockIdx.y = 0) or Uinv
+// (blockIdx.y = 1).  From L Linv = I, top to bottom:
+//   Linv(i, j) = -Dinv_i sum_{k=j}^{i-1} L(i, k) Linv(k, j),   Linv(j, j) = Dinv_j;
+// from U Uinv = I, bottom to top:
+//   Uinv(i, j) = -Einv_i sum_{k=i+1}^{j} U(i, k) Uinv(k, j),   Uinv(j, j) = Einv_j,
+// with Dinv, Einv the 32 x 32 sub-block inverses.  The four waves split the
+// sum over k and add their partial products in LDS.
+__global__ __launch_bounds__(256) void tri_inv_kernel(int b, const float *__restrict__ LU, int64_t lda,
+                                                      const float *__restrict__ inv32, float *__restrict__ tri)
+{
+    __shared__ float W[kPanelMax * kXS];   // W[r * kXS + c] = inverse(r, 32 j + c)
+    __shared__ float P[4 * 1024];          // per-wave partial sums, lane-major
+    __shared__ float S[kStrip * kXS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, lh = lane >> 5;
+    const int nb = b / kStrip, j = blockIdx.x;
+    const bool upper = blockIdx.y != 0;
+    const int half = upper ? 1024 : 0;
+    {
+        const float *dj = inv32 + static_cast<int64_t>(j) * kInvBlock + half;   // row-major 32 x 32
+        for (int e = tid; e < 1024; e += 256) {
+            const int r = e / 32, c = e % 32;
+            const float v = dj[e];
+            W[(32 * j + r) * kXS + c] = v;
+            tri_store(tri, nb, upper, j, j, r, c, v);
+        }
+    }
+    __syncthreads();
+    const int steps = upper ? j : nb - 1 - j;
+    for (int st = 1; st <= steps; ++st) {
+        const int i = upper ? j - st : j + st;
+        const int klo = upper ? i + 1 : j, khi = upper ? j : i - 1;
+        floatx16t_t acc;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+        for (int k = klo + wave; k <= khi; k += 4) {
+            // A = L(i, k) or U(i, k): element (32 i + l32, 32 k + 2 kk + lh)
+            const float *src = LU + static_cast<int64_t>(32 * k + lh) * lda + 32 * i + l32;
+            float av[16];
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk) av[kk] = src[static_cast<int64_t>(2 * kk) * lda];
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], W[(32 * k + 2 * kk + lh) * kXS + l32], acc, 0,
+                                                           0, 0);
+        }
+#pragma unroll
+        for (int e = 0; e < 16; ++e) P[wave * 1024 + lane * 16 + e] = acc[e];
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int e = lane * 16 + 4 * q + r;
+                    S[(8 * q + 4 * lh + r) * kXS + l32] = P[e] + P[1024 + e] + P[2048 + e] + P[3072 + e];
+                }
+        }
+        __syncthreads();
+        if (wave == 0) {
+            float av[16];
+            load_k_pairs(inv32 + static_cast<int64_t>(i) * kInvBlock + half + l32 * 32, lh, av);
+            floatx16t_t r2;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) r2[e] = 0.0f;
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+                r2 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], S[(2 * kk + lh) * kXS + l32], r2, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int rho = 8 * q + 4 * lh + r;
+                    const float v = -r2[4 * q + r];
+                    W[(32 * i + rho) * kXS + l32] = v;
+                    tri_store(tri, nb, upper, i, j, rho, l32, v);
+                }
+        }
+        __syncthreads();
+    }
+}
+
+// Snake assignment of output blocks to the four waves, largest first: block
+// nb - 1 - p goes to wave snake(p), so every wave gets about the same number
+// of 32 x 32 x 32 products (block q costs q + 1 of them).
+__device__ __forceinline__ int tri_snake(int p) { return ((p >> 2) & 1) ? 3 - (p & 3) : (p & 3); }
+
+// L(m x b) := L Ukk^-1 (update_left_panels, panel_trf.hpp:143-157) as
+// L := L Uinv: output block jb = sum_{kb <= jb} X(:, kb) Uinv(kb, jb).
+template <bool VEC>
+__global__ __launch_bounds__(256) void trsm_l_tri_kernel(int64_t m, int b, const float *__restrict__ tri,
+                                                         float *__restrict__ L, int64_t ldl,
+                                                         float *__restrict__ amax, int tb)
 {
     __shared__ float X[kPanelMax * kXS];   // X[c * kXS + r]: column c, row i0 + r
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -277,60 +391,61 @@ __global__ __launch_bounds__(256) void trsm_l_mfma_kernel(int64_t m, int b, cons
         }
     }
     __syncthreads();
-    for (int jb = 0; jb < b; jb += kStrip) {
-        // X(:, jb:jb+32) := X(:, jb:jb+32) Uinv  (wave 0)
-        if (wave == 0) {
-            const float *Ui = inv + static_cast<int64_t>(jb / kStrip) * kInvBlock + 1024;   // row-major
-            floatx16t_t acc;
+    const int nb = b / kStrip;
+    const float *Up = tri + static_cast<int64_t>(nb) * nb * 1024;
+    float vmax = 0.0f;
+    for (int p = 0; p < nb; ++p) {
+        if (tri_snake(p) != wave) continue;
+        const int jb = nb - 1 - p;
+        const float4 *bp = reinterpret_cast<const float4 *>(Up + static_cast<int64_t>(jb) * nb * 1024) + lane * 4;
+        floatx16t_t acc;
 #pragma unroll
-            for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+        for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+        float4 cur[4];
 #pragma unroll
-            for (int kk = 0; kk < 16; ++kk) {
-                const int k = 2 * kk + lh;
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(X[(jb + k) * kXS + l32], Ui[k * 32 + l32], acc, 0, 0, 0);
-            }
+        for (int e = 0; e < 4; ++e) cur[e] = bp[e];
+        for (int kb = 0; kb <= jb; ++kb) {
+            float4 nxt[4];
+            const float4 *bn = bp + static_cast<int64_t>(min(kb + 1, jb)) * 256;
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) X[(jb + l32) * kXS + 8 * q + 4 * lh + r] = acc[4 * q + r];
-        }
-        __syncthreads();
-        // X(:, c0:c0+32) -= X(:, jb:jb+32) U(jb:jb+32, c0:c0+32) for the blocks right of jb
-        const int nblk = (b - jb - kStrip) / kStrip;
-        for (int blk = wave; blk < nblk; blk += 4) {
-            const int c0 = jb + kStrip + blk * kStrip;
-            float bv[16];
-            load_k_pairs(Uk + jb + static_cast<int64_t>(c0 + l32) * ldu, lh, bv);
-            floatx16t_t acc;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+            for (int e = 0; e < 4; ++e) nxt[e] = bn[e];
+            const float bv[16] = {cur[0].x, cur[0].y, cur[0].z, cur[0].w, cur[1].x, cur[1].y, cur[1].z, cur[1].w,
+                                  cur[2].x, cur[2].y, cur[2].z, cur[2].w, cur[3].x, cur[3].y, cur[3].z, cur[3].w};
 #pragma unroll
             for (int kk = 0; kk < 16; ++kk)
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(X[(jb + 2 * kk + lh) * kXS + l32], bv[kk], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(X[(32 * kb + 2 * kk + lh) * kXS + l32], bv[kk], acc, 0,
+                                                           0, 0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) cur[e] = nxt[e];
+        }
+        float *dst = L + static_cast<int64_t>(32 * jb + l32) * ldl + i0 + 4 * lh;
+        if (VEC && nr == kStrip) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float4 o = {acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+                *reinterpret_cast<float4 *>(dst + 8 * q) = o;
+                vmax = fmaxf(vmax, fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fmaxf(fabsf(o.z), fabsf(o.w))));
+            }
+        } else {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) X[(c0 + l32) * kXS + 8 * q + 4 * lh + r] -= acc[4 * q + r];
-        }
-        __syncthreads();
-    }
-    float vmax = 0.0f;
-    for (int idx = tid; idx < b * kStrip; idx += 256) {
-        const int c = idx / kStrip, r = idx % kStrip;
-        if (r < nr) {
-            const float v = X[c * kXS + r];
-            L[static_cast<int64_t>(c) * ldl + i0 + r] = v;
-            vmax = fmaxf(vmax, fabsf(v));
+                for (int r = 0; r < 4; ++r)
+                    if (8 * q + 4 * lh + r < nr) {
+                        dst[8 * q + r] = acc[4 * q + r];
+                        vmax = fmaxf(vmax, fabsf(acc[4 * q + r]));
+                    }
         }
     }
     tile_max_reduce(amax, i0 / tb, vmax, true);
 }
 
-// U(b x n) := Lkk^-1 U (update_right_panels, panel_trf.hpp:158-171), b % 32 == 0.
-__global__ __launch_bounds__(256) void trsm_u_mfma_kernel(int b, int64_t n, const float *__restrict__ Lk,
-                                                          int64_t ldl, const float *__restrict__ inv,
-                                                          float *__restrict__ U, int64_t ldu,
-                                                          float *__restrict__ amax, int tb)
+// U(b x n) := Lkk^-1 U (update_right_panels, panel_trf.hpp:158-171) as
+// U := Linv U: output block rb = sum_{kb <= rb} Linv(rb, kb) X(kb, :).
+template <bool VEC>
+__global__ __launch_bounds__(256) void trsm_u_tri_kernel(int b, int64_t n, const float *__restrict__ tri,
+                                                         float *__restrict__ U, int64_t ldu,
+                                                         float *__restrict__ amax, int tb)
 {
     __shared__ float X[kPanelMax * kXS];   // X[r * kXS + c]: row r of column j0 + c
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -354,51 +469,47 @@ __global__ __launch_bounds__(256) void trsm_u_mfma_kernel(int b, int64_t n, cons
         }
     }
     __syncthreads();
-    for (int kb = 0; kb < b; kb += kStrip) {
-        // X(kb:kb+32, :) := Linv X(kb:kb+32, :)  (wave 0)
-        if (wave == 0) {
-            const float *Li = inv + static_cast<int64_t>(kb / kStrip) * kInvBlock;   // row-major Linv
-            float av[16];
-            load_k_pairs(Li + l32 * 32, lh, av);
-            floatx16t_t acc;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
-#pragma unroll
-            for (int kk = 0; kk < 16; ++kk)
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], X[(kb + 2 * kk + lh) * kXS + l32], acc, 0, 0, 0);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) X[(kb + 8 * q + 4 * lh + r) * kXS + l32] = acc[4 * q + r];
-        }
-        __syncthreads();
-        // X(r0:r0+32, :) -= L(r0:r0+32, kb:kb+32) X(kb:kb+32, :) for the blocks below kb
-        const int nblk = (b - kb - kStrip) / kStrip;
-        for (int blk = wave; blk < nblk; blk += 4) {
-            const int r0 = kb + kStrip + blk * kStrip;
-            float av[16];
-#pragma unroll
-            for (int kk = 0; kk < 16; ++kk) av[kk] = Lk[(r0 + l32) + static_cast<int64_t>(kb + 2 * kk + lh) * ldl];
-            floatx16t_t acc;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
-#pragma unroll
-            for (int kk = 0; kk < 16; ++kk)
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], X[(kb + 2 * kk + lh) * kXS + l32], acc, 0, 0, 0);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) X[(r0 + 8 * q + 4 * lh + r) * kXS + l32] -= acc[4 * q + r];
-        }
-        __syncthreads();
-    }
+    const int nb = b / kStrip;
+    const bool col_ok = l32 < nc;
     float vmax = 0.0f;
-    for (int idx = tid; idx < b * kStrip; idx += 256) {
-        const int c = idx / b, r = idx % b;
-        if (c < nc) {
-            const float v = X[r * kXS + c];
-            U[(j0 + c) * ldu + r] = v;
-            vmax = fmaxf(vmax, fabsf(v));
+    for (int p = 0; p < nb; ++p) {
+        if (tri_snake(p) != wave) continue;
+        const int rb = nb - 1 - p;
+        const float4 *ap = reinterpret_cast<const float4 *>(tri + static_cast<int64_t>(rb) * nb * 1024) + lane * 4;
+        floatx16t_t acc;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+        float4 cur[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) cur[e] = ap[e];
+        for (int kb = 0; kb <= rb; ++kb) {
+            float4 nxt[4];
+            const float4 *an = ap + static_cast<int64_t>(min(kb + 1, rb)) * 256;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) nxt[e] = an[e];
+            const float av[16] = {cur[0].x, cur[0].y, cur[0].z, cur[0].w, cur[1].x, cur[1].y, cur[1].z, cur[1].w,
+                                  cur[2].x, cur[2].y, cur[2].z, cur[2].w, cur[3].x, cur[3].y, cur[3].z, cur[3].w};
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], X[(32 * kb + 2 * kk + lh) * kXS + l32], acc, 0,
+                                                           0, 0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) cur[e] = nxt[e];
+        }
+        if (col_ok) {
+            float *dst = U + (j0 + l32) * ldu + 32 * rb + 4 * lh;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (VEC) {
+                    const float4 o = {acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+                    *reinterpret_cast<float4 *>(dst + 8 * q) = o;
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) dst[8 * q + r] = acc[4 * q + r];
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) vmax = fmaxf(vmax, fabsf(acc[4 * q + r]));
+            }
         }
     }
     tile_max_reduce(amax, j0 / tb, vmax, true);
@@ -411,25 +522,45 @@ bool trsm_strip_ok(int b, const float *amax, int tb)
     return b <= kPanelMax && (amax == nullptr || (tb % kStrip) == 0);
 }
 
-int64_t diag_inv_floats(int b) { return ceil_div(b, kStrip) * kInvBlock; }
+static bool tri_ok(int b) { return b % kStrip == 0 && b <= kPanelMax; }
+
+int64_t diag_inv_floats(int b)
+{
+    const int64_t nb = ceil_div(b, kStrip);
+    return nb * kInvBlock + (tri_ok(b) ? 2 * nb * nb * 1024 : 0);
+}
 
 int launch_diag_inv(int b, const float *LU, int64_t lda, float *inv, hipStream_t s)
 {
-    hipLaunchKernelGGL(diag_inv32_kernel, dim3(static_cast<unsigned>(ceil_div(b, kStrip))), dim3(256), 0,
-                       s, b, LU, lda, inv);
+    const int nb = static_cast<int>(ceil_div(b, kStrip));
+    hipLaunchKernelGGL(diag_inv32_kernel, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, b, LU, lda, inv);
     HPLAI_HIP(hipGetLastError());
+    if (tri_ok(b)) {
+        hipLaunchKernelGGL(tri_inv_kernel, dim3(static_cast<unsigned>(nb), 2), dim3(256), 0, s, b, LU, lda, inv,
+                           inv + static_cast<int64_t>(nb) * kInvBlock);
+        HPLAI_HIP(hipGetLastError());
+    }
     return HPLAI_OK;
+}
+
+static bool aligned16(const float *p, int64_t ld)
+{
+    return (ld % 4) == 0 && (reinterpret_cast<uintptr_t>(p) % 16) == 0;
 }
 
 int launch_trsm_l_strip(int64_t m, int b, const float *U, int64_t ldu, const float *inv, float *L,
                         int64_t ldl, float *amax, int tb, hipStream_t s)
 {
-    if (b % kStrip == 0 && (ldu % 4) == 0 && (reinterpret_cast<uintptr_t>(U) % 16) == 0)
-        hipLaunchKernelGGL(trsm_l_mfma_kernel, dim3(static_cast<unsigned>(ceil_div(m, kStrip))), dim3(256),
-                           0, s, m, b, U, ldu, inv, L, ldl, amax, tb);
-    else
-        hipLaunchKernelGGL(trsm_l_strip_kernel, dim3(static_cast<unsigned>(ceil_div(m, kStrip))), dim3(256),
-                           0, s, m, b, U, ldu, inv, L, ldl, amax, tb);
+    const dim3 grid(static_cast<unsigned>(ceil_div(m, kStrip)));
+    if (tri_ok(b)) {
+        const float *tri = inv + ceil_div(b, kStrip) * kInvBlock;
+        if (aligned16(L, ldl))
+            hipLaunchKernelGGL(trsm_l_tri_kernel<true>, grid, dim3(256), 0, s, m, b, tri, L, ldl, amax, tb);
+        else
+            hipLaunchKernelGGL(trsm_l_tri_kernel<false>, grid, dim3(256), 0, s, m, b, tri, L, ldl, amax, tb);
+    } else {
+        hipLaunchKernelGGL(trsm_l_strip_kernel, grid, dim3(256), 0, s, m, b, U, ldu, inv, L, ldl, amax, tb);
+    }
     HPLAI_HIP(hipGetLastError());
     return HPLAI_OK;
 }
@@ -437,12 +568,16 @@ int launch_trsm_l_strip(int64_t m, int b, const float *U, int64_t ldu, const flo
 int launch_trsm_u_strip(int b, int64_t n, const float *Lkk, int64_t ldl, const float *inv, float *U,
                         int64_t ldu, float *amax, int tb, hipStream_t s)
 {
-    if (b % kStrip == 0)
-        hipLaunchKernelGGL(trsm_u_mfma_kernel, dim3(static_cast<unsigned>(ceil_div(n, kStrip))), dim3(256),
-                           0, s, b, n, Lkk, ldl, inv, U, ldu, amax, tb);
-    else
-        hipLaunchKernelGGL(trsm_u_strip_kernel, dim3(static_cast<unsigned>(ceil_div(n, kStrip))), dim3(256),
-                           0, s, b, n, Lkk, ldl, inv, U, ldu, amax, tb);
+    const dim3 grid(static_cast<unsigned>(ceil_div(n, kStrip)));
+    if (tri_ok(b)) {
+        const float *tri = inv + ceil_div(b, kStrip) * kInvBlock;
+        if (aligned16(U, ldu))
+            hipLaunchKernelGGL(trsm_u_tri_kernel<true>, grid, dim3(256), 0, s, b, n, tri, U, ldu, amax, tb);
+        else
+            hipLaunchKernelGGL(trsm_u_tri_kernel<false>, grid, dim3(256), 0, s, b, n, tri, U, ldu, amax, tb);
+    } else {
+        hipLaunchKernelGGL(trsm_u_strip_kernel, grid, dim3(256), 0, s, b, n, Lkk, ldl, inv, U, ldu, amax, tb);
+    }
     HPLAI_HIP(hipGetLastError());
     return HPLAI_OK;
 }

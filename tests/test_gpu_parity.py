@@ -110,7 +110,8 @@ def test_getrf_zero_pivot_reported(ctx):
     assert int(info.cpu()[0]) == 6
 
 
-@pytest.mark.parametrize("m,b", [(1140, 60), (3584, 512), (777, 100), (1000, 128), (40000, 512)])
+@pytest.mark.parametrize("m,b", [(1140, 60), (3584, 512), (777, 100), (1000, 128), (40000, 512), (1001, 512),
+                                 (1001, 96), (333, 224)])
 def test_trsm_lpanel(ctx, oracle, m, b):
     n = m + b
     D = _diag_block(oracle, n, b, 0)
@@ -125,7 +126,8 @@ def test_trsm_lpanel(ctx, oracle, m, b):
     assert rel < 2e-5, rel
 
 
-@pytest.mark.parametrize("b,ncol", [(60, 1140), (512, 3584), (100, 777), (128, 1000), (512, 40000)])
+@pytest.mark.parametrize("b,ncol", [(60, 1140), (512, 3584), (100, 777), (128, 1000), (512, 40000), (512, 1001),
+                                    (96, 1001), (224, 333)])
 def test_trsm_upanel(ctx, oracle, b, ncol):
     n = ncol + b
     D = _diag_block(oracle, n, b, 0)
