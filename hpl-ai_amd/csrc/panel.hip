@@ -172,95 +172,177 @@ __global__ __launch_bounds__(kGetrfThreads) void getrf_kernel(int n, float *__re
 
 // ---------------------------------------------------------------------------
 // The same blocked LU for n <= 512 with n % 32 == 0 and 16-byte aligned
-// columns, with the trailing update on v_mfma_f32_32x32x2_f32 (FP32 inputs,
-// FP32 accumulation): each wave updates 32 x 32 tiles of A22, four tiles at a
-// time with their sixteen float4 loads per lane in flight together.  L21 and
-// U12 come from LDS (column-major L21, row-major U12).
+// columns, built around the FP32 MFMA (v_mfma_f32_32x32x2_f32, FP32 inputs
+// and accumulation).  Per 32-column step k:
+//  1. the h x 32 panel and the 32 x (n - k - 32) U strip are staged in LDS;
+//  2. wave 0 factors the 32 x 32 diagonal block in registers (lane i holds
+//     row i; pivot rows are broadcast with v_readlane, no barriers);
+//  3. wave 0 inverts the unit-lower L11, wave 1 the upper U11 (Gauss-Jordan
+//     on the rows, again broadcast by readlane);
+//  4. every wave takes 32 x 32 tiles of L21 := A21 U11^-1 and
+//     U12 := L11^-1 A12 on the MFMA;
+//  5. every wave updates 32 x 32 tiles of A22 -= L21 U12 (C in global memory,
+//     four tiles with their sixteen float4 loads in flight together).
+// L21 and U12 come from LDS: Lc column-major (stride kLcS), Us column-major
+// (stride 33, one column of the strip per 33 floats).
 // ---------------------------------------------------------------------------
 typedef float floatx16g_t __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float readlane_f(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+constexpr int kLcS = 512 + 4;   // Lc column stride (float4 column writes spread over the banks)
 
 __global__ __launch_bounds__(kGetrfThreads) void getrf_mfma_kernel(int n, float *__restrict__ A, int64_t lda,
                                                                    int *__restrict__ info)
 {
-    constexpr int NT = kGetrfThreads, NMAX = 512, W = 32, RPT = NMAX / NT;
-    __shared__ __attribute__((aligned(16))) float Lc[W * NMAX];   // panel, Lc[c * NMAX + r]
-    __shared__ __attribute__((aligned(16))) float Us[W * NMAX];   // U strip, Us[r * NMAX + c]
-    __shared__ float prow[2][W];
-    __shared__ float Mi[32 * 33], Ti[32 * 33];
+    constexpr int NT = kGetrfThreads, NMAX = 512, W = 32;
+    __shared__ __attribute__((aligned(16))) float Lc[W * kLcS];          // panel, Lc[c * kLcS + r]
+    __shared__ __attribute__((aligned(16))) float Us[(NMAX - W) * 33];   // U strip, Us[c * 33 + r]
+    __shared__ float Li[32 * 33], Ui[32 * 33];                           // L11^-1, U11^-1 row-major
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l32 = lane & 31, lh = lane >> 5;
     int bad = 0;
     for (int k = 0; k < n; k += W) {
-        const int h = n - k;
+        const int h = n - k, wr = n - k - W;
         float *Ak = A + static_cast<int64_t>(k) * lda + k;
-        float p[RPT][W];
+        // 1. stage the panel (h x 32) and the U strip (32 x wr)
+        for (int e = tid; e < h * (W / 4); e += NT) {
+            const int c = e / (h / 4), r4 = (e % (h / 4)) * 4;
+            const float4 v = *reinterpret_cast<const float4 *>(Ak + static_cast<int64_t>(c) * lda + r4);
+            *reinterpret_cast<float4 *>(&Lc[c * kLcS + r4]) = v;
+        }
+        for (int e = tid; e < wr * W; e += NT) {
+            const int c = e / W, r = e % W;
+            Us[c * 33 + r] = Ak[static_cast<int64_t>(W + c) * lda + r];
+        }
+        __syncthreads();
+        // 2. LU of the 32 x 32 diagonal block by wave 0
+        if (wave == 0) {
+            const int i = l32;
+            float a[W];
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            const int r = tid + q * NT;
+            for (int j = 0; j < W; ++j) a[j] = Lc[j * kLcS + i];
+#pragma unroll
+            for (int kk = 0; kk < W; ++kk) {
+                const float piv = readlane_f(a[kk], kk);
+                if (bad == 0 && !(piv != 0.0f && isfinite(piv))) bad = k + kk + 1;
+                const float inv = 1.0f / piv;
+                const bool below = i > kk;
+                const float l = a[kk] * inv;
+                a[kk] = below ? l : a[kk];
+#pragma unroll
+                for (int j = kk + 1; j < W; ++j) {
+                    const float ukj = readlane_f(a[j], kk);
+                    a[j] = below ? fmaf(-l, ukj, a[j]) : a[j];
+                }
+            }
+            if (lh == 0) {
+#pragma unroll
+                for (int j = 0; j < W; ++j) {
+                    Lc[j * kLcS + i] = a[j];
+                    Ak[static_cast<int64_t>(j) * lda + i] = a[j];
+                }
+            }
+        }
+        __syncthreads();
+        if (wr <= 0) break;
+        // 3. L11^-1 (wave 0) and U11^-1 (wave 1), Gauss-Jordan on rows held one per lane
+        if (wave == 0) {
+            const int i = l32;
+            float a[W], x[W];
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                a[j] = Lc[j * kLcS + i];
+                x[j] = j == i ? 1.0f : 0.0f;
+            }
+            // rows i > c: X(i, :) -= L(i, c) X(c, :), X(c, :) final at step c
 #pragma unroll
             for (int c = 0; c < W; ++c) {
-                const float v = Ak[static_cast<int64_t>(c) * lda + min(r, h - 1)];
-                p[q][c] = r < h ? v : 0.0f;
-            }
-        }
-        // unblocked LU of the h x 32 panel, one barrier per pivot
+                const bool below = i > c;
 #pragma unroll
-        for (int kk = 0; kk < W; ++kk) {
-            if (tid == kk)
-#pragma unroll
-                for (int c = 0; c < W; ++c) prow[kk & 1][c] = p[0][c];
-            __syncthreads();
-            const float piv = prow[kk & 1][kk];
-            const float inv = 1.0f / piv;
-            if (tid == 0 && bad == 0 && !(piv != 0.0f && isfinite(piv))) bad = k + kk + 1;
-#pragma unroll
-            for (int q = 0; q < RPT; ++q) {
-                const int r = tid + q * NT;
-                if (r > kk && r < h) {
-                    const float l = p[q][kk] * inv;
-                    p[q][kk] = l;
-#pragma unroll
-                    for (int c = kk + 1; c < W; ++c) p[q][c] = p[q][c] - l * prow[kk & 1][c];
+                for (int j = 0; j <= c; ++j) {
+                    const float xcj = readlane_f(x[j], c);
+                    x[j] = below ? fmaf(-a[c], xcj, x[j]) : x[j];
                 }
             }
-            __builtin_amdgcn_sched_barrier(0);
-        }
+            if (lh == 0)
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            const int r = tid + q * NT;
-            if (r < h)
+                for (int j = 0; j < W; ++j) Li[i * 33 + j] = x[j];
+        } else if (wave == 1) {
+            const int i = l32;
+            float a[W], y[W];
 #pragma unroll
-                for (int c = 0; c < W; ++c) {
-                    Ak[static_cast<int64_t>(c) * lda + r] = p[q][c];
-                    Lc[c * NMAX + r] = p[q][c];
-                }
-        }
-        __syncthreads();
-        const int wr = n - k - W;
-        if (wr <= 0) break;
-        // U strip: x := L11^-1 x through the inverse of the unit-lower L11
-        for (int e = tid; e < 32 * 32; e += NT) {
-            const int r = e / 32, c = e % 32;
-            Mi[r * 33 + c] = r > c ? Lc[c * NMAX + r] : (r == c ? 1.0f : 0.0f);
-        }
-        __syncthreads();
-        lower_inv32(Mi, Ti, tid, NT);
-        for (int c = tid; c < wr; c += NT) {
-            float x[W];
-            float *col = Ak + static_cast<int64_t>(W + c) * lda;
-#pragma unroll
-            for (int r = 0; r < W; ++r) x[r] = col[r];
-            for (int i = 0; i < W; ++i) {
-                float acc = 0.0f;
-#pragma unroll
-                for (int r = 0; r < W; ++r) acc = fmaf(Mi[i * 33 + r], x[r], acc);
-                col[i] = acc;
-                Us[i * NMAX + c] = acc;
+            for (int j = 0; j < W; ++j) {
+                a[j] = Lc[j * kLcS + i];
+                y[j] = j == i ? 1.0f : 0.0f;
             }
+            // from the bottom: Y(c, :) /= U(c, c), then rows i < c: Y(i, :) -= U(i, c) Y(c, :)
+#pragma unroll
+            for (int c = W - 1; c >= 0; --c) {
+                const float rd = 1.0f / readlane_f(a[c], c);
+                if (i == c)
+#pragma unroll
+                    for (int j = c; j < W; ++j) y[j] *= rd;
+                const bool above = i < c;
+#pragma unroll
+                for (int j = c; j < W; ++j) {
+                    const float ycj = readlane_f(y[j], c);
+                    y[j] = above ? fmaf(-a[c], ycj, y[j]) : y[j];
+                }
+            }
+            if (lh == 0)
+#pragma unroll
+                for (int j = 0; j < W; ++j) Ui[i * 33 + j] = y[j];
         }
         __syncthreads();
-        // trailing update A22 -= L21 U12 (hr x wr, both multiples of 32)
+        // 4. L21 := A21 U11^-1 (TR tiles) and U12 := L11^-1 A12 (TC tiles)
         const int TR = (h - W) / 32, TC = wr / 32;
+        for (int t = wave; t < TR + TC; t += NT / 64) {
+            floatx16g_t acc;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+            if (t < TR) {
+                const int r0 = W + 32 * t;
+                float av[16], bv[16];
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk) {
+                    av[kk] = Lc[(2 * kk + lh) * kLcS + r0 + l32];
+                    bv[kk] = Ui[(2 * kk + lh) * 33 + l32];
+                }
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], bv[kk], acc, 0, 0, 0);
+                float *g = Ak + static_cast<int64_t>(l32) * lda + r0 + 4 * lh;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float4 o = {acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+                    *reinterpret_cast<float4 *>(&Lc[l32 * kLcS + r0 + 8 * q + 4 * lh]) = o;
+                    *reinterpret_cast<float4 *>(g + 8 * q) = o;
+                }
+            } else {
+                const int c0 = 32 * (t - TR);
+                float av[16], bv[16];
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk) {
+                    av[kk] = Li[l32 * 33 + 2 * kk + lh];
+                    bv[kk] = Us[(c0 + l32) * 33 + 2 * kk + lh];
+                }
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], bv[kk], acc, 0, 0, 0);
+                float *g = Ak + static_cast<int64_t>(W + c0 + l32) * lda + 4 * lh;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float4 o = {acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) Us[(c0 + l32) * 33 + 8 * q + 4 * lh + r] = acc[4 * q + r];
+                    *reinterpret_cast<float4 *>(g + 8 * q) = o;
+                }
+            }
+        }
+        __syncthreads();
+        // 5. trailing update A22 -= L21 U12 (hr x wr, both multiples of 32)
         for (int t0 = wave * 4; t0 < TR * TC; t0 += 4 * (NT / 64)) {
             float4 cv[4][4];
 #pragma unroll
@@ -282,8 +364,8 @@ __global__ __launch_bounds__(kGetrfThreads) void getrf_mfma_kernel(int n, float 
 #pragma unroll
                 for (int kk = 0; kk < 16; ++kk) {
                     const int kr = 2 * kk + lh;
-                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Lc[kr * NMAX + W + ti * 32 + l32],
-                                                               Us[kr * NMAX + tj * 32 + l32], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Lc[kr * kLcS + W + ti * 32 + l32],
+                                                               Us[(tj * 32 + l32) * 33 + kr], acc, 0, 0, 0);
                 }
                 float *ccol = Ak + static_cast<int64_t>(W + tj * 32 + l32) * lda + W + ti * 32 + 4 * lh;
 #pragma unroll
