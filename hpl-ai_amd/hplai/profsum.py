@@ -3,10 +3,18 @@
     python -m hplai.profsum stats  KERNEL_STATS_CSV [--top N]
     python -m hplai.profsum traffic --fetch FETCH_CSV --write WRITE_CSV
                                     --kernel NAME_SUBSTRING --copy-bytes BYTES
+    python -m hplai.profsum launches KERNEL_TRACE_CSV --kernel NAME_SUBSTRING
+                                    [--first N]
 
 `stats` reads the `*_kernel_stats.csv` written by `rocprofv3 --kernel-trace
 --stats --output-format csv` and prints one line per kernel (calls, total and
 average duration, share of GPU time).
+
+`launches` reads the `*_kernel_trace.csv` of the same run and prints the
+count and mean duration of the first N launches (in dispatch order) of the
+kernel whose name contains NAME_SUBSTRING; with N = the launches of one solve
+this is the figure to set beside the bench line's live per-launch time when
+the run holds more than one solve.
 
 `traffic` reads two `*_counter_collection.csv` files written by
 `rocprofv3 --pmc FETCH_SIZE` and `rocprofv3 --pmc WRITE_SIZE` over the same
@@ -90,6 +98,20 @@ def traffic(fetch, write, kernel, copy_bytes):
     print("hbm_bytes_per_launch %.6g" % (fetch_per + write_per))
 
 
+def launches(path, kernel, first):
+    rows = [r for r in _rows(path) if kernel in r["Kernel_Name"]]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    if first > 0:
+        rows = rows[:first]
+    if not rows:
+        raise SystemExit("kernel %r not found" % kernel)
+    durs = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in rows]
+    print("kernel %s" % kernel)
+    print("launches %d" % len(durs))
+    print("mean_ms %.4f" % (sum(durs) / len(durs) * 1e-6))
+    print("total_ms %.3f" % (sum(durs) * 1e-6))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m hplai.profsum")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -101,9 +123,15 @@ def main(argv=None):
     t.add_argument("--write", required=True)
     t.add_argument("--kernel", required=True)
     t.add_argument("--copy-bytes", type=int, required=True)
+    g = sub.add_parser("launches")
+    g.add_argument("csv")
+    g.add_argument("--kernel", required=True)
+    g.add_argument("--first", type=int, default=0)
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         stats(a.csv, a.top)
+    elif a.cmd == "launches":
+        launches(a.csv, a.kernel, a.first)
     else:
         traffic(a.fetch, a.write, a.kernel, a.copy_bytes)
 
