@@ -74,6 +74,13 @@ int launch_otf_gemv(int64_t n, uint64_t seed, const double *d, double alpha, con
 int64_t otf_gemv_work_doubles(int64_t n);
 // triangular solves with the FP32 factors of LU, FP64 vectors; nrhs columns
 // of x with stride ldx
+bool tri_inverse_ok(int b);
+int64_t tri_inverse_offset(int b);
+int launch_tri_trsv(bool upper, int b, const float *tri, const double *xin, double *xout, hipStream_t s);
+int launch_trsv_lower_inv(int64_t n, int bs, const float *LU, int64_t lda, double *x, const float *inv,
+                          int64_t inv_stride, double *xsc, hipStream_t s);
+int launch_trsv_upper_inv(int64_t n, int bs, const float *LU, int64_t lda, double *x, const float *inv,
+                          int64_t inv_stride, double *xsc, hipStream_t s);
 int launch_trsv_lower(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s,
                       int nrhs = 1, int64_t ldx = 0);   // x := L^-1 x
 int launch_trsv_upper(int64_t n, int bs, const float *LU, int64_t lda, double *x, hipStream_t s,

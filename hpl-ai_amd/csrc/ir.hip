@@ -127,15 +127,22 @@ static void launch_trsv_diag(int w, const float *T, int64_t lda, double *x, int6
 // in a fixed order through LDS.  Right-hand side blockIdx.y at offset y * ldx.
 constexpr int kGemvRowsT = 64;
 
+// With xcopy != nullptr (one right-hand side), workgroup 0 also stores xb
+// into xcopy(0:w): the solved segment goes from its scratch copy to x while
+// the update runs.
 __global__ __launch_bounds__(256) void trsv_gemv_kernel(int64_t m, int w, const float *__restrict__ P,
                                                         int64_t lda, const double *__restrict__ xb,
-                                                        double *__restrict__ y, int64_t ldx)
+                                                        double *__restrict__ y, int64_t ldx,
+                                                        double *__restrict__ xcopy)
 {
     __shared__ double xs[kTrsvMax];
     __shared__ double part[4][kGemvRowsT];
     xb += static_cast<int64_t>(blockIdx.y) * ldx;
     y += static_cast<int64_t>(blockIdx.y) * ldx;
-    for (int c = threadIdx.x; c < w; c += 256) xs[c] = xb[c];
+    for (int c = threadIdx.x; c < w; c += 256) {
+        xs[c] = xb[c];
+        if (xcopy != nullptr && blockIdx.x == 0) xcopy[c] = xb[c];
+    }
     __syncthreads();
     const int lr = threadIdx.x % kGemvRowsT, g = threadIdx.x / kGemvRowsT;
     const int64_t r = static_cast<int64_t>(blockIdx.x) * kGemvRowsT + lr;
@@ -189,14 +196,57 @@ __global__ __launch_bounds__(256) void trsv_gemv_t_kernel(int64_t m, int w, cons
 }
 
 static void launch_gemv(bool trans, int64_t m, int w, const float *P, int64_t lda, const double *xb, double *y,
-                        int64_t ldx, int nrhs, hipStream_t s)
+                        int64_t ldx, int nrhs, hipStream_t s, double *xcopy = nullptr)
 {
     if (trans)
         hipLaunchKernelGGL(trsv_gemv_t_kernel, dim3(static_cast<unsigned>(ceil_div(m, kGemvTRows)), nrhs),
                            dim3(256), 0, s, m, w, P, lda, xb, y, ldx);
     else
         hipLaunchKernelGGL(trsv_gemv_kernel, dim3(static_cast<unsigned>(ceil_div(m, kGemvRowsT)), nrhs),
-                           dim3(256), 0, s, m, w, P, lda, xb, y, ldx);
+                           dim3(256), 0, s, m, w, P, lda, xb, y, ldx, xcopy);
+}
+
+// The same block solves with the diagonal blocks applied through the inverses
+// the LU stored (tri_inv_kernel layout, one set per block at inv + q
+// inv_stride + tri_inverse_offset(bs)): x_q := Tinv_q r_q into the scratch
+// segment xsc (bs doubles), then the update of the other rows from xsc, whose
+// workgroup 0 also moves x_q into x.  One right-hand side; n % bs == 0 and
+// tri_inverse_ok(bs).
+int launch_trsv_lower_inv(int64_t n, int bs, const float *LU, int64_t lda, double *x, const float *inv,
+                          int64_t inv_stride, double *xsc, hipStream_t s)
+{
+    HPLAI_CHECK(tri_inverse_ok(bs) && n % bs == 0, "trsv: no stored inverses for bs=%d", bs);
+    const int64_t off = tri_inverse_offset(bs);
+    for (int64_t kb = 0; kb < n; kb += bs) {
+        int rc = launch_tri_trsv(false, bs, inv + (kb / bs) * inv_stride + off, x + kb, xsc, s);
+        if (rc) return rc;
+        const int64_t m = n - kb - bs;
+        if (m > 0) {
+            launch_gemv(false, m, bs, LU + kb * lda + kb + bs, lda, xsc, x + kb + bs, 0, 1, s, x + kb);
+            HPLAI_HIP(hipGetLastError());
+        } else {
+            HPLAI_HIP(hipMemcpyAsync(x + kb, xsc, sizeof(double) * bs, hipMemcpyDeviceToDevice, s));
+        }
+    }
+    return HPLAI_OK;
+}
+
+int launch_trsv_upper_inv(int64_t n, int bs, const float *LU, int64_t lda, double *x, const float *inv,
+                          int64_t inv_stride, double *xsc, hipStream_t s)
+{
+    HPLAI_CHECK(tri_inverse_ok(bs) && n % bs == 0, "trsv: no stored inverses for bs=%d", bs);
+    const int64_t off = tri_inverse_offset(bs);
+    for (int64_t kb = n - bs; kb >= 0; kb -= bs) {
+        int rc = launch_tri_trsv(true, bs, inv + (kb / bs) * inv_stride + off, x + kb, xsc, s);
+        if (rc) return rc;
+        if (kb > 0) {
+            launch_gemv(false, kb, bs, LU + kb * lda, lda, xsc, x, 0, 1, s, x + kb);
+            HPLAI_HIP(hipGetLastError());
+        } else {
+            HPLAI_HIP(hipMemcpyAsync(x + kb, xsc, sizeof(double) * bs, hipMemcpyDeviceToDevice, s));
+        }
+    }
+    return HPLAI_OK;
 }
 
 // x := L^-1 x (unit lower L of LU), nrhs columns of x with stride ldx

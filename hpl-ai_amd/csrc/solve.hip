@@ -47,6 +47,7 @@ struct Buffers {
     int *info = nullptr;
     double *chk = nullptr;
     float *inv = nullptr;   // diagonal sub-block inverses, one set per diagonal block
+    double *tsc = nullptr;  // one block of the IR's triangular solves (scratch)
 };
 
 void free_buffers(Buffers &B, bool keep_factors)
@@ -61,6 +62,7 @@ void free_buffers(Buffers &B, bool keep_factors)
     f(B.d);
     f(B.rhs);
     f(B.r);
+    f(B.tsc);
     f(B.work);
     f(B.norms);
     f(B.info);
@@ -331,7 +333,9 @@ struct LU {
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_DIAG_LU, 2.0 / 3.0 * bd * bd * bd, s, [&] {
                     int r = launch_getrf(b, a(c, c), lda, B->info + c / b, s);
-                    if (r == HPLAI_OK && mrow > 0 && b <= 512)
+                    // the inverses also serve the IR's block solves, so the
+                    // last diagonal block gets them too
+                    if (r == HPLAI_OK && b <= 512)
                         r = launch_diag_inv(b, a(c, c), lda, const_cast<float *>(dinv(c)), s);
                     return r;
                 });
@@ -642,6 +646,7 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     ALLOC(B.rhs, sizeof(double) * n);
     ALLOC(B.x, sizeof(double) * n);
     ALLOC(B.r, sizeof(double) * n);
+    ALLOC(B.tsc, sizeof(double) * b);
     ALLOC(B.work, sizeof(double) * otf_gemv_work_doubles(n));
     ALLOC(B.norms, sizeof(double) * 4);
     ALLOC(B.chk, sizeof(double) * 3);
@@ -713,6 +718,13 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
                 break;
             }
             rc = lu.timed(HPLAI_PHASE_IR_TRSV, 2.0 * dn2, s, [&] {
+                // diagonal blocks through the inverses the LU stored, where it stored them
+                if (tri_inverse_ok(b)) {
+                    const int64_t st = diag_inv_floats(b);
+                    int r = launch_trsv_lower_inv(n, b, B.A, lu.lda, B.r, B.inv, st, B.tsc, s);
+                    if (r == HPLAI_OK) r = launch_trsv_upper_inv(n, b, B.A, lu.lda, B.r, B.inv, st, B.tsc, s);
+                    return r;
+                }
                 int r = launch_trsv_lower(n, b, B.A, lu.lda, B.r, s);
                 if (r == HPLAI_OK) r = launch_trsv_upper(n, b, B.A, lu.lda, B.r, s);
                 return r;

@@ -543,6 +543,77 @@ int launch_diag_inv(int b, const float *LU, int64_t lda, float *inv, hipStream_t
     return HPLAI_OK;
 }
 
+// x_out := Tinv x_in for one factored diagonal block (the IR's diagonal-block
+// solve as a product with the stored inverse): Linv (UPPER = false, the unit
+// lower solve) or Uinv (UPPER = true).  Workgroup I computes rows
+// 32 I .. 32 I + 31; its 256 threads are 32 rows x 8 column-block groups,
+// FP32 inverse entries times the FP64 vector summed in FP64, the 8 partial
+// sums of a row added in a fixed order.  Out of place: every workgroup reads
+// the whole input segment.
+template <bool UPPER>
+__global__ __launch_bounds__(256) void tri_trsv_kernel(int b, const float *__restrict__ tri,
+                                                       const double *__restrict__ xin, double *__restrict__ xout)
+{
+    __shared__ double xs[kPanelMax];
+    __shared__ double part[8][kStrip];
+    const int tid = threadIdx.x, rho = tid & 31, p = tid >> 5;
+    const int nb = b / kStrip, I = blockIdx.x;
+    for (int c = tid; c < b; c += 256) xs[c] = xin[c];
+    __syncthreads();
+    double acc = 0.0;
+    if (!UPPER) {
+        for (int J = p; J <= I; J += 8) {
+            const float *blk = tri + (static_cast<int64_t>(I) * nb + J) * 1024;
+            const float4 *ev = reinterpret_cast<const float4 *>(blk + rho * 16);          // gamma = 2 kk
+            const float4 *od = reinterpret_cast<const float4 *>(blk + (rho + 32) * 16);   // gamma = 2 kk + 1
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float4 a = ev[e], c = od[e];
+                const double *xj = xs + 32 * J + 8 * e;
+                acc = fma(static_cast<double>(a.x), xj[0], acc);
+                acc = fma(static_cast<double>(c.x), xj[1], acc);
+                acc = fma(static_cast<double>(a.y), xj[2], acc);
+                acc = fma(static_cast<double>(c.y), xj[3], acc);
+                acc = fma(static_cast<double>(a.z), xj[4], acc);
+                acc = fma(static_cast<double>(c.z), xj[5], acc);
+                acc = fma(static_cast<double>(a.w), xj[6], acc);
+                acc = fma(static_cast<double>(c.w), xj[7], acc);
+            }
+        }
+    } else {
+        const float *Up = tri + static_cast<int64_t>(nb) * nb * 1024;
+        for (int J = I + p; J < nb; J += 8) {
+            const float *blk = Up + (static_cast<int64_t>(J) * nb + I) * 1024 + 32 * 16 * (rho & 1) + (rho >> 1);
+            const double *xj = xs + 32 * J;
+#pragma unroll
+            for (int g = 0; g < 32; ++g) acc = fma(static_cast<double>(blk[g * 16]), xj[g], acc);
+        }
+    }
+    part[p][rho] = acc;
+    __syncthreads();
+    if (p == 0) {
+        double y = part[0][rho];
+#pragma unroll
+        for (int q = 1; q < 8; ++q) y += part[q][rho];
+        xout[32 * I + rho] = y;
+    }
+}
+
+bool tri_inverse_ok(int b) { return tri_ok(b); }
+
+int64_t tri_inverse_offset(int b) { return ceil_div(b, kStrip) * kInvBlock; }
+
+int launch_tri_trsv(bool upper, int b, const float *tri, const double *xin, double *xout, hipStream_t s)
+{
+    const dim3 grid(static_cast<unsigned>(b / kStrip));
+    if (upper)
+        hipLaunchKernelGGL(tri_trsv_kernel<true>, grid, dim3(256), 0, s, b, tri, xin, xout);
+    else
+        hipLaunchKernelGGL(tri_trsv_kernel<false>, grid, dim3(256), 0, s, b, tri, xin, xout);
+    HPLAI_HIP(hipGetLastError());
+    return HPLAI_OK;
+}
+
 static bool aligned16(const float *p, int64_t ld)
 {
     return (ld % 4) == 0 && (reinterpret_cast<uintptr_t>(p) % 16) == 0;

@@ -337,6 +337,23 @@ def test_solve_outer_blocked_matches_oracle(ctx, oracle, outer, lookahead, lazy)
     assert np.max(np.abs(x - xr)) / np.max(np.abs(xr)) < 1e-10
 
 
+def test_solve_ir_through_stored_inverses(ctx, oracle):
+    """b % 32 == 0: the IR's diagonal-block solves are products with the
+    inverses the LU stored (tri_trsv_kernel) instead of substitution.  The
+    refined x must still match the oracle's (FP64 substitution with the
+    oracle's factors) and converge in as many steps."""
+    n, b = 4096, 128
+    res = ctx.solve(n, b, fmt="f16", outer=4, keep_factors=True)
+    lu = ctx.solution_factors(n)
+    x = ctx.solution_x(n)
+    ctx.release()
+    ref, d = oracle.lu(oracle.LU_PRECOMP, n, b, SEED)
+    orc = oracle.ir(ref, n, d)
+    assert res["converged"] and res["harness"] < 16.0 and res["ir_steps"] <= 4, res
+    assert np.max(np.abs(lu - ref)) / np.max(np.abs(ref)) < 1e-4
+    assert np.max(np.abs(x - orc["x"])) / np.max(np.abs(orc["x"])) < 1e-10
+
+
 def test_solve_c1_lazy(ctx):
     res = ctx.solve(1200, 60, fmt="f16", lazy=True)
     assert res["converged"] and res["harness"] < 16.0 and res["ir_steps"] <= 4
