@@ -65,6 +65,7 @@ int hplai_create(int device, hplai_ctx *out)
     }
     c->num_cu = prop.multiProcessorCount;
     e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->bulk, hipStreamNonBlocking);
     for (int i = 0; e == hipSuccess && i < 8; ++i)
         e = hipEventCreateWithFlags(&c->ev[i], hipEventDisableTiming);
     if (e != hipSuccess) {
@@ -83,6 +84,7 @@ int hplai_destroy(hplai_ctx ctx)
     for (int i = 0; i < 8; ++i)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
+    if (ctx->bulk) (void)hipStreamDestroy(ctx->bulk);
     delete ctx;
     return HPLAI_OK;
 }

@@ -10,6 +10,7 @@ struct hplai_ctx_s {
     int device = 0;
     int num_cu = 0;
     hipStream_t side = nullptr;    // second stream for look-ahead panel work
+    hipStream_t bulk = nullptr;    // third stream: the panel's rows below the outer block
     hipEvent_t ev[8] = {};
     // buffers kept by hplai_solve(keep_factors)
     float *A = nullptr;

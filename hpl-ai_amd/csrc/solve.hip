@@ -242,7 +242,7 @@ struct LU {
     uint64_t seed;
     int64_t ntiles;
     Buffers *B;
-    hipStream_t main, side;
+    hipStream_t main, side, bulk;
     hipEvent_t *ev;
     int num_cu;
     GemmTimer *timer;
@@ -311,25 +311,44 @@ struct LU {
         });
     }
 
-    // Step 1 for the outer block at c0 with sw inner blocks.
+    // Step 1 for the outer block at c0 with sw inner blocks.  Two streams:
+    // s carries the chain from one diagonal block to the next (getrf, the
+    // inverses, the TRSMs and quantization of the rows and columns inside the
+    // outer block, the in-block update); the rows below the outer block (their
+    // left-looking update, lazy initialisation, TRSM and quantization) go to
+    // the bulk stream, which waits for the chain only where it needs its
+    // results.  s waits for the bulk stream at the end.
     int panel(int64_t c0, int sw, int buf, hipStream_t s) const
     {
         int rc = HPLAI_OK;
         const int64_t Bw = static_cast<int64_t>(sw) * b;
+        const int64_t rb0 = c0 + Bw, mb = n - rb0;   // rows below the outer block
+        hipStream_t q = bulk;
+        hipEvent_t ev_fork = ev[3], ev_inv = ev[4], ev_u = ev[5], ev_join = ev[6];
         if (tiled()) {
             rc = launch_fill_f32(B->amaxL[buf], static_cast<int64_t>(outer) * ntiles, 0.0f, s);
             if (rc == HPLAI_OK) rc = launch_fill_f32(B->amaxU[buf], static_cast<int64_t>(outer) * ntiles, 0.0f, s);
             if (rc) return rc;
         }
+        if (mb > 0) {
+            HPLAI_HIP(hipEventRecord(ev_fork, s));
+            HPLAI_HIP(hipStreamWaitEvent(q, ev_fork, 0));
+        }
         for (int t = 0; t < sw && rc == HPLAI_OK; ++t) {
             const int64_t c = c0 + static_cast<int64_t>(t) * b;
             const double bd = static_cast<double>(b);
-            // rows below the outer block take all earlier panels of the block
-            // at once (left-looking, K = t b); the rows inside the block were
-            // updated panel by panel below
-            if (t > 0 && n - (c0 + Bw) > 0) rc = gemm(buf, 0, t, c0 + Bw, c, n - (c0 + Bw), b, s);
-            if (rc == HPLAI_OK) rc = lazy_init(c, c, n - c, b, s);   // block column: diagonal block + L panel
-            const int64_t mrow = n - c - b;
+            // bulk: the rows below take all earlier panels of the block at once
+            // (left-looking, K = t b) once the chain has quantized U of block
+            // t - 1 (the wait refers to the record of the previous iteration)
+            if (mb > 0) {
+                if (t > 0) {
+                    HPLAI_HIP(hipStreamWaitEvent(q, ev_u, 0));
+                    rc = gemm(buf, 0, t, rb0, c, mb, b, q);
+                }
+                if (rc == HPLAI_OK) rc = lazy_init(rb0, c, mb, b, q);
+            }
+            // chain: the diagonal block and the rows of the block column inside the outer block
+            if (rc == HPLAI_OK) rc = lazy_init(c, c, rb0 - c, b, s);
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_DIAG_LU, 2.0 / 3.0 * bd * bd * bd, s, [&] {
                     int r = launch_getrf(b, a(c, c), lda, B->info + c / b, s);
@@ -339,7 +358,25 @@ struct LU {
                         r = launch_diag_inv(b, a(c, c), lda, const_cast<float *>(dinv(c)), s);
                     return r;
                 });
-            if (rc || mrow <= 0) continue;
+            if (rc) break;
+            // bulk: L := A U^-1 and quantization of the rows below
+            if (mb > 0) {
+                HPLAI_HIP(hipEventRecord(ev_inv, s));
+                HPLAI_HIP(hipStreamWaitEvent(q, ev_inv, 0));
+                rc = timed(HPLAI_PHASE_TRSM_L, static_cast<double>(mb) * bd * bd, q, [&] {
+                    return launch_trsm_l(mb, b, a(c, c), lda, a(rb0, c), lda, aml(buf, t, rb0), b,
+                                         b <= 512 ? dinv(c) : nullptr, q);
+                });
+                if (rc == HPLAI_OK)
+                    rc = timed(HPLAI_PHASE_CONV_L, static_cast<double>(mb) * bd, q, [&] {
+                        return launch_quantize(fmt, 0, mb, b, b, a(rb0, c), lda, lq(buf, rb0, t), ldq,
+                                               aml(buf, t, rb0), el(buf, t, rb0), q);
+                    });
+                if (rc) break;
+            }
+            // chain: the same for the rows inside the outer block
+            const int64_t mrow = rb0 - c - b;
+            if (mrow <= 0) continue;
             rc = timed(HPLAI_PHASE_TRSM_L, static_cast<double>(mrow) * bd * bd, s, [&] {
                 return launch_trsm_l(mrow, b, a(c, c), lda, a(c + b, c), lda, aml(buf, t, c + b), b,
                                      b <= 512 ? dinv(c) : nullptr, s);
@@ -349,8 +386,8 @@ struct LU {
                     return launch_quantize(fmt, 0, mrow, b, b, a(c + b, c), lda, lq(buf, c + b, t), ldq,
                                            aml(buf, t, c + b), el(buf, t, c + b), s);
                 });
-            const int64_t ncols = c0 + Bw - (c + b);
-            if (rc || ncols <= 0) continue;
+            const int64_t ncols = rb0 - (c + b);
+            if (rc) break;
             rc = lazy_init(c, c + b, b, ncols, s);   // block row inside the panel
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_TRSM_R, static_cast<double>(ncols) * bd * bd, s, [&] {
@@ -362,9 +399,15 @@ struct LU {
                     return launch_quantize(fmt, 1, ncols, b, b, a(c, c + b), lda, uq(buf, c + b, t), ldq,
                                            amu(buf, t, c + b), eu(buf, t, c + b), s);
                 });
+            if (rc) break;
+            if (mb > 0) HPLAI_HIP(hipEventRecord(ev_u, s));
             // right-looking inside the outer block only (rows and columns
             // c + b .. c0 + Bw, K = b)
-            if (rc == HPLAI_OK) rc = gemm(buf, t, 1, c + b, c + b, ncols, ncols, s);
+            rc = gemm(buf, t, 1, c + b, c + b, ncols, ncols, s);
+        }
+        if (mb > 0) {
+            HPLAI_HIP(hipEventRecord(ev_join, q));
+            HPLAI_HIP(hipStreamWaitEvent(s, ev_join, 0));
         }
         return rc;
     }
@@ -467,6 +510,7 @@ void init_lu(LU &lu, hplai_ctx ctx, int64_t n, int64_t lda, int b, int fmt, int 
     lu.B = B;
     lu.main = nullptr;
     lu.side = ctx->side;
+    lu.bulk = ctx->bulk;
     lu.ev = ctx->ev;
     lu.num_cu = ctx->num_cu;
 }
