@@ -357,50 +357,58 @@ __global__ __launch_bounds__(256) void tri_inv_kernel(int b, const float *__rest
     }
 }
 
-// Snake assignment of output blocks to the four waves, largest first: block
+// Snake assignment of output blocks to the eight waves, largest first: block
 // nb - 1 - p goes to wave snake(p), so every wave gets about the same number
-// of 32 x 32 x 32 products (block q costs q + 1 of them).
-__device__ __forceinline__ int tri_snake(int p) { return ((p >> 2) & 1) ? 3 - (p & 3) : (p & 3); }
+// of 32 x 32 x 32 products (block q costs q + 1 of them; with nb = 16 every
+// wave gets 17).
+__device__ __forceinline__ int tri_snake(int p) { return ((p >> 3) & 1) ? 7 - (p & 7) : (p & 7); }
+
+constexpr int kTriStrip = 64;              // rows (L) or columns (U) per workgroup
+constexpr int kTriXS = kTriStrip + 1;      // LDS stride
+constexpr int kTriThreads = 512;
 
 // L(m x b) := L Ukk^-1 (update_left_panels, panel_trf.hpp:143-157) as
-// L := L Uinv: output block jb = sum_{kb <= jb} X(:, kb) Uinv(kb, jb).
+// L := L Uinv: output block jb = sum_{kb <= jb} X(:, kb) Uinv(kb, jb).  A
+// workgroup owns 64 rows (two 32-row halves sharing every Uinv operand).
 template <bool VEC>
-__global__ __launch_bounds__(256) void trsm_l_tri_kernel(int64_t m, int b, const float *__restrict__ tri,
-                                                         float *__restrict__ L, int64_t ldl,
-                                                         float *__restrict__ amax, int tb)
+__global__ __launch_bounds__(kTriThreads) void trsm_l_tri_kernel(int64_t m, int b, const float *__restrict__ tri,
+                                                                 float *__restrict__ L, int64_t ldl,
+                                                                 float *__restrict__ amax, int tb)
 {
-    __shared__ float X[kPanelMax * kXS];   // X[c * kXS + r]: column c, row i0 + r
+    __shared__ float X[kPanelMax * kTriXS];   // X[c * kTriXS + r]: column c, row i0 + r
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l32 = lane & 31, lh = lane >> 5;
-    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kStrip;
-    const int nr = m - i0 < kStrip ? static_cast<int>(m - i0) : kStrip;
-    const int tot = b * kStrip;
-    for (int base = 0; base < tot; base += 256 * 16) {
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kTriStrip;
+    const int nr = m - i0 < kTriStrip ? static_cast<int>(m - i0) : kTriStrip;
+    const int tot = b * kTriStrip;
+    for (int base = 0; base < tot; base += kTriThreads * 16) {
         float v[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
-            const int idx = min(base + u * 256 + tid, tot - 1);
-            const int c = idx / kStrip, r = idx % kStrip;
+            const int idx = min(base + u * kTriThreads + tid, tot - 1);
+            const int c = idx / kTriStrip, r = idx % kTriStrip;
             v[u] = L[static_cast<int64_t>(c) * ldl + i0 + min(r, nr - 1)];
         }
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
-            const int idx = base + u * 256 + tid;
-            const int c = idx / kStrip, r = idx % kStrip;
-            if (idx < tot) X[c * kXS + r] = r < nr ? v[u] : 0.0f;
+            const int idx = base + u * kTriThreads + tid;
+            const int c = idx / kTriStrip, r = idx % kTriStrip;
+            if (idx < tot) X[c * kTriXS + r] = r < nr ? v[u] : 0.0f;
         }
     }
     __syncthreads();
     const int nb = b / kStrip;
     const float *Up = tri + static_cast<int64_t>(nb) * nb * 1024;
-    float vmax = 0.0f;
+    float vmax[2] = {0.0f, 0.0f};
     for (int p = 0; p < nb; ++p) {
         if (tri_snake(p) != wave) continue;
         const int jb = nb - 1 - p;
         const float4 *bp = reinterpret_cast<const float4 *>(Up + static_cast<int64_t>(jb) * nb * 1024) + lane * 4;
-        floatx16t_t acc;
+        floatx16t_t acc[2];
 #pragma unroll
-        for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[h][e] = 0.0f;
         float4 cur[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) cur[e] = bp[e];
@@ -412,73 +420,82 @@ __global__ __launch_bounds__(256) void trsm_l_tri_kernel(int64_t m, int b, const
             const float bv[16] = {cur[0].x, cur[0].y, cur[0].z, cur[0].w, cur[1].x, cur[1].y, cur[1].z, cur[1].w,
                                   cur[2].x, cur[2].y, cur[2].z, cur[2].w, cur[3].x, cur[3].y, cur[3].z, cur[3].w};
 #pragma unroll
-            for (int kk = 0; kk < 16; ++kk)
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(X[(32 * kb + 2 * kk + lh) * kXS + l32], bv[kk], acc, 0,
-                                                           0, 0);
+            for (int kk = 0; kk < 16; ++kk) {
+                const float *xr = X + (32 * kb + 2 * kk + lh) * kTriXS + l32;
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr[0], bv[kk], acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr[32], bv[kk], acc[1], 0, 0, 0);
+            }
 #pragma unroll
             for (int e = 0; e < 4; ++e) cur[e] = nxt[e];
         }
-        float *dst = L + static_cast<int64_t>(32 * jb + l32) * ldl + i0 + 4 * lh;
-        if (VEC && nr == kStrip) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float4 o = {acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
-                *reinterpret_cast<float4 *>(dst + 8 * q) = o;
-                vmax = fmaxf(vmax, fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fmaxf(fabsf(o.z), fabsf(o.w))));
+        for (int h = 0; h < 2; ++h) {
+            float *dst = L + static_cast<int64_t>(32 * jb + l32) * ldl + i0 + 32 * h + 4 * lh;
+            if (VEC && nr == kTriStrip) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float4 o = {acc[h][4 * q], acc[h][4 * q + 1], acc[h][4 * q + 2], acc[h][4 * q + 3]};
+                    *reinterpret_cast<float4 *>(dst + 8 * q) = o;
+                    vmax[h] = fmaxf(vmax[h], fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fmaxf(fabsf(o.z), fabsf(o.w))));
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (32 * h + 8 * q + 4 * lh + r < nr) {
+                            dst[8 * q + r] = acc[h][4 * q + r];
+                            vmax[h] = fmaxf(vmax[h], fabsf(acc[h][4 * q + r]));
+                        }
             }
-        } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    if (8 * q + 4 * lh + r < nr) {
-                        dst[8 * q + r] = acc[4 * q + r];
-                        vmax = fmaxf(vmax, fabsf(acc[4 * q + r]));
-                    }
         }
     }
-    tile_max_reduce(amax, i0 / tb, vmax, true);
+    // the two halves may fall in different tiles when tb % 64 == 32
+    tile_max_reduce(amax, i0 / tb, vmax[0], true);
+    tile_max_reduce(amax, (i0 + 32) / tb, vmax[1], 32 < nr);
 }
 
 // U(b x n) := Lkk^-1 U (update_right_panels, panel_trf.hpp:158-171) as
-// U := Linv U: output block rb = sum_{kb <= rb} Linv(rb, kb) X(kb, :).
+// U := Linv U: output block rb = sum_{kb <= rb} Linv(rb, kb) X(kb, :).  A
+// workgroup owns 64 columns (two 32-column halves sharing every Linv operand).
 template <bool VEC>
-__global__ __launch_bounds__(256) void trsm_u_tri_kernel(int b, int64_t n, const float *__restrict__ tri,
-                                                         float *__restrict__ U, int64_t ldu,
-                                                         float *__restrict__ amax, int tb)
+__global__ __launch_bounds__(kTriThreads) void trsm_u_tri_kernel(int b, int64_t n, const float *__restrict__ tri,
+                                                                 float *__restrict__ U, int64_t ldu,
+                                                                 float *__restrict__ amax, int tb)
 {
-    __shared__ float X[kPanelMax * kXS];   // X[r * kXS + c]: row r of column j0 + c
+    __shared__ float X[kPanelMax * kTriXS];   // X[r * kTriXS + c]: row r of column j0 + c
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l32 = lane & 31, lh = lane >> 5;
-    const int64_t j0 = static_cast<int64_t>(blockIdx.x) * kStrip;
-    const int nc = n - j0 < kStrip ? static_cast<int>(n - j0) : kStrip;
-    const int tot = b * kStrip;
-    for (int base = 0; base < tot; base += 256 * 16) {
+    const int64_t j0 = static_cast<int64_t>(blockIdx.x) * kTriStrip;
+    const int nc = n - j0 < kTriStrip ? static_cast<int>(n - j0) : kTriStrip;
+    const int tot = b * kTriStrip;
+    for (int base = 0; base < tot; base += kTriThreads * 16) {
         float v[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
-            const int idx = min(base + u * 256 + tid, tot - 1);
+            const int idx = min(base + u * kTriThreads + tid, tot - 1);
             const int c = idx / b, r = idx % b;
             v[u] = U[(j0 + min(c, nc - 1)) * ldu + r];
         }
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
-            const int idx = base + u * 256 + tid;
+            const int idx = base + u * kTriThreads + tid;
             const int c = idx / b, r = idx % b;
-            if (idx < tot) X[r * kXS + c] = c < nc ? v[u] : 0.0f;
+            if (idx < tot) X[r * kTriXS + c] = c < nc ? v[u] : 0.0f;
         }
     }
     __syncthreads();
     const int nb = b / kStrip;
-    const bool col_ok = l32 < nc;
-    float vmax = 0.0f;
+    float vmax[2] = {0.0f, 0.0f};
     for (int p = 0; p < nb; ++p) {
         if (tri_snake(p) != wave) continue;
         const int rb = nb - 1 - p;
         const float4 *ap = reinterpret_cast<const float4 *>(tri + static_cast<int64_t>(rb) * nb * 1024) + lane * 4;
-        floatx16t_t acc;
+        floatx16t_t acc[2];
 #pragma unroll
-        for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[h][e] = 0.0f;
         float4 cur[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) cur[e] = ap[e];
@@ -490,29 +507,34 @@ __global__ __launch_bounds__(256) void trsm_u_tri_kernel(int b, int64_t n, const
             const float av[16] = {cur[0].x, cur[0].y, cur[0].z, cur[0].w, cur[1].x, cur[1].y, cur[1].z, cur[1].w,
                                   cur[2].x, cur[2].y, cur[2].z, cur[2].w, cur[3].x, cur[3].y, cur[3].z, cur[3].w};
 #pragma unroll
-            for (int kk = 0; kk < 16; ++kk)
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], X[(32 * kb + 2 * kk + lh) * kXS + l32], acc, 0,
-                                                           0, 0);
+            for (int kk = 0; kk < 16; ++kk) {
+                const float *xr = X + (32 * kb + 2 * kk + lh) * kTriXS + l32;
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], xr[0], acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], xr[32], acc[1], 0, 0, 0);
+            }
 #pragma unroll
             for (int e = 0; e < 4; ++e) cur[e] = nxt[e];
         }
-        if (col_ok) {
-            float *dst = U + (j0 + l32) * ldu + 32 * rb + 4 * lh;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (32 * h + l32 >= nc) continue;
+            float *dst = U + (j0 + 32 * h + l32) * ldu + 32 * rb + 4 * lh;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 if (VEC) {
-                    const float4 o = {acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+                    const float4 o = {acc[h][4 * q], acc[h][4 * q + 1], acc[h][4 * q + 2], acc[h][4 * q + 3]};
                     *reinterpret_cast<float4 *>(dst + 8 * q) = o;
                 } else {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) dst[8 * q + r] = acc[4 * q + r];
+                    for (int r = 0; r < 4; ++r) dst[8 * q + r] = acc[h][4 * q + r];
                 }
 #pragma unroll
-                for (int r = 0; r < 4; ++r) vmax = fmaxf(vmax, fabsf(acc[4 * q + r]));
+                for (int r = 0; r < 4; ++r) vmax[h] = fmaxf(vmax[h], fabsf(acc[h][4 * q + r]));
             }
         }
     }
-    tile_max_reduce(amax, j0 / tb, vmax, true);
+    tile_max_reduce(amax, j0 / tb, vmax[0], true);
+    tile_max_reduce(amax, (j0 + 32) / tb, vmax[1], 32 < nc);
 }
 
 // The strip kernels need b <= 512 and, when tile maxima are collected, every
@@ -625,10 +647,11 @@ int launch_trsm_l_strip(int64_t m, int b, const float *U, int64_t ldu, const flo
     const dim3 grid(static_cast<unsigned>(ceil_div(m, kStrip)));
     if (tri_ok(b)) {
         const float *tri = inv + ceil_div(b, kStrip) * kInvBlock;
+        const dim3 g64(static_cast<unsigned>(ceil_div(m, kTriStrip)));
         if (aligned16(L, ldl))
-            hipLaunchKernelGGL(trsm_l_tri_kernel<true>, grid, dim3(256), 0, s, m, b, tri, L, ldl, amax, tb);
+            hipLaunchKernelGGL(trsm_l_tri_kernel<true>, g64, dim3(kTriThreads), 0, s, m, b, tri, L, ldl, amax, tb);
         else
-            hipLaunchKernelGGL(trsm_l_tri_kernel<false>, grid, dim3(256), 0, s, m, b, tri, L, ldl, amax, tb);
+            hipLaunchKernelGGL(trsm_l_tri_kernel<false>, g64, dim3(kTriThreads), 0, s, m, b, tri, L, ldl, amax, tb);
     } else {
         hipLaunchKernelGGL(trsm_l_strip_kernel, grid, dim3(256), 0, s, m, b, U, ldu, inv, L, ldl, amax, tb);
     }
@@ -642,10 +665,11 @@ int launch_trsm_u_strip(int b, int64_t n, const float *Lkk, int64_t ldl, const f
     const dim3 grid(static_cast<unsigned>(ceil_div(n, kStrip)));
     if (tri_ok(b)) {
         const float *tri = inv + ceil_div(b, kStrip) * kInvBlock;
+        const dim3 g64(static_cast<unsigned>(ceil_div(n, kTriStrip)));
         if (aligned16(U, ldu))
-            hipLaunchKernelGGL(trsm_u_tri_kernel<true>, grid, dim3(256), 0, s, b, n, tri, U, ldu, amax, tb);
+            hipLaunchKernelGGL(trsm_u_tri_kernel<true>, g64, dim3(kTriThreads), 0, s, b, n, tri, U, ldu, amax, tb);
         else
-            hipLaunchKernelGGL(trsm_u_tri_kernel<false>, grid, dim3(256), 0, s, b, n, tri, U, ldu, amax, tb);
+            hipLaunchKernelGGL(trsm_u_tri_kernel<false>, g64, dim3(kTriThreads), 0, s, b, n, tri, U, ldu, amax, tb);
     } else {
         hipLaunchKernelGGL(trsm_u_strip_kernel, grid, dim3(256), 0, s, b, n, Lkk, ldl, inv, U, ldu, amax, tb);
     }
