@@ -45,6 +45,16 @@ __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + 
 int launch_gen_matrix(int64_t n, uint64_t seed, int64_t i0, int64_t j0, int64_t m, int64_t k,
                       const double *d, float *A, int64_t lda, int accumulate, hipStream_t s);
 int launch_gen_diag(int64_t n, uint64_t seed, int64_t i0, int64_t cnt, double *d, hipStream_t s);
+// Higham's HPL-AI matrix (higham.hip)
+int higham_coeffs(int64_t n, double cond, double rho, double *alpha, double *beta);
+int launch_gen_higham(double alpha, double beta, int64_t i0, int64_t j0, int64_t m, int64_t k, float *A,
+                      int64_t lda, int accumulate, hipStream_t s);
+int launch_higham_diag(double alpha, double beta, int64_t i0, int64_t cnt, double *d, hipStream_t s);
+int64_t higham_gemv_work_doubles(int64_t n);
+int launch_higham_gemv(int64_t n, double alpha, double beta, double scale, const double *x, double *y,
+                       double *work, hipStream_t s);
+int launch_higham_norm(int64_t n, double alpha, double beta, double *ones, double *rows, double *work,
+                       double *out, hipStream_t s);
 int launch_gen_rhs(int64_t n, uint64_t seed, int64_t i0, int64_t cnt, double *b, hipStream_t s);
 
 int launch_getrf(int n, float *A, int64_t lda, int *info, hipStream_t s);

@@ -239,6 +239,8 @@ struct LU {
     int64_t n, lda, ldq;
     int b, outer, fmt, kpad, es;
     bool lazy;
+    bool higham = false;          // Higham's matrix instead of the LCG matrix
+    double h_alpha = 0.0, h_beta = 0.0;
     uint64_t seed;
     int64_t ntiles;
     Buffers *B;
@@ -307,6 +309,7 @@ struct LU {
     {
         if (!lazy || m <= 0 || k <= 0) return HPLAI_OK;
         return timed(HPLAI_PHASE_LAZY_INIT, static_cast<double>(m) * k, s, [&] {
+            if (higham) return launch_gen_higham(h_alpha, h_beta, i0, j0, m, k, a(i0, j0), lda, 1, s);
             return launch_gen_matrix(n, seed, i0, j0, m, k, B->d, a(i0, j0), lda, 1, s);
         });
     }
@@ -669,6 +672,20 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     GemmTimer timer;
     timer.on = p->profile != 0;
     init_lu(lu, ctx, n, (n + 63) / 64 * 64, b, fmt, outer, p->lazy != 0, p->seed, &B, &timer);
+    HPLAI_CHECK(p->matrix == HPLAI_MATRIX_HPLAI || p->matrix == HPLAI_MATRIX_HIGHAM ||
+                    p->matrix == HPLAI_MATRIX_HIGHAM_KAPPA,
+                "hplai_solve: unknown matrix family %d", p->matrix);
+    if (p->matrix != HPLAI_MATRIX_HPLAI) {
+        // HMGen(n, 1000., 0.5, d), main.cpp:258-259: the reference's signs
+        // (+alpha, +beta), or Fasi and Higham's (-alpha, -beta) with cond = kappa
+        lu.higham = true;
+        rc = higham_coeffs(n, p->cond > 0.0 ? p->cond : 1000.0, 0.5, &lu.h_alpha, &lu.h_beta);
+        if (rc) return rc;
+        if (p->matrix == HPLAI_MATRIX_HIGHAM_KAPPA) {
+            lu.h_alpha = -lu.h_alpha;
+            lu.h_beta = -lu.h_beta;
+        }
+    }
     EpochLog epochs;
     epochs.on = p->verbose != 0;
     epochs.n = n;
@@ -691,7 +708,7 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     ALLOC(B.x, sizeof(double) * n);
     ALLOC(B.r, sizeof(double) * n);
     ALLOC(B.tsc, sizeof(double) * b);
-    ALLOC(B.work, sizeof(double) * otf_gemv_work_doubles(n));
+    ALLOC(B.work, sizeof(double) * std::max(otf_gemv_work_doubles(n), higham_gemv_work_doubles(n)));
     ALLOC(B.norms, sizeof(double) * 4);
     ALLOC(B.chk, sizeof(double) * 3);
 #undef ALLOC
@@ -704,12 +721,15 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     // ---- generation (main.cpp:256-275), outside the timed region ----
     HPLAI_HIP(hipDeviceSynchronize());
     double t0 = now_s();
-    if (rc == HPLAI_OK) rc = launch_gen_diag(n, p->seed, 0, n, B.d, s);
+    if (rc == HPLAI_OK)
+        rc = lu.higham ? launch_higham_diag(lu.h_alpha, lu.h_beta, 0, n, B.d, s)   // pdiaggen(HMGen)
+                       : launch_gen_diag(n, p->seed, 0, n, B.d, s);
     if (lu.lazy) {
         // pmatgen0 (matgen.hpp:80-105): partial sums start at zero
         HPLAI_HIP(hipMemsetAsync(B.A, 0, sizeof(float) * lu.lda * n, s));
     } else if (rc == HPLAI_OK) {
-        rc = launch_gen_matrix(n, p->seed, 0, 0, n, n, B.d, B.A, lu.lda, 0, s);
+        rc = lu.higham ? launch_gen_higham(lu.h_alpha, lu.h_beta, 0, 0, n, n, B.A, lu.lda, 0, s)
+                       : launch_gen_matrix(n, p->seed, 0, 0, n, n, B.d, B.A, lu.lda, 0, s);
     }
     if (rc == HPLAI_OK) rc = launch_gen_rhs(n, p->seed, 0, n, B.rhs, s);
     HPLAI_HIP(hipDeviceSynchronize());
@@ -732,17 +752,25 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     if (rc == HPLAI_OK) {
         double hn[4];
         rc = launch_norm_inf(n, B.rhs, B.norms + 0, s);
-        if (rc == HPLAI_OK) rc = launch_norm_inf(n, B.d, B.norms + 1, s);
+        if (rc == HPLAI_OK)   // x and r are scratch until x0 is set
+            rc = lu.higham ? launch_higham_norm(n, lu.h_alpha, lu.h_beta, B.x, B.r, B.work, B.norms + 1, s)
+                           : launch_norm_inf(n, B.d, B.norms + 1, s);
         if (rc == HPLAI_OK) rc = launch_vec_div(n, B.rhs, B.d, B.x, s);   // x0 = b ./ diag(A)
         HPLAI_HIP(hipMemcpy(hn, B.norms, sizeof(double) * 2, hipMemcpyDeviceToHost));
-        const double normb = hn[0], norma = 2.0 * hn[1];   // hpl_infnorm, panel_norm.hpp:78-84
+        // hpl_infnorm (panel_norm.hpp:78-84): 2 max d for the LCG matrix; the
+        // Higham matrix's largest row sum (higham_infnorm, panel_norm.hpp:87-135)
+        const double normb = hn[0], norma = lu.higham ? hn[1] : 2.0 * hn[1];
         for (int it = 0; it < maxit && rc == HPLAI_OK; ++it) {
             HPLAI_HIP(hipMemcpyAsync(B.r, B.rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
             rc = launch_norm_inf(n, B.x, B.norms + 2, s);
             const double dn2 = static_cast<double>(n) * static_cast<double>(n);
             if (rc == HPLAI_OK)
                 rc = lu.timed(HPLAI_PHASE_IR_GEMV, 2.0 * dn2, s,
-                              [&] { return launch_otf_gemv(n, p->seed, B.d, -1.0, B.x, B.r, B.work, s); });
+                              [&] {
+                                  if (lu.higham)
+                                      return launch_higham_gemv(n, lu.h_alpha, lu.h_beta, -1.0, B.x, B.r, B.work, s);
+                                  return launch_otf_gemv(n, p->seed, B.d, -1.0, B.x, B.r, B.work, s);
+                              });
             if (rc == HPLAI_OK) rc = launch_norm_inf(n, B.r, B.norms + 3, s);
             if (rc) break;
             HPLAI_HIP(hipMemcpy(hn + 2, B.norms + 2, sizeof(double) * 2, hipMemcpyDeviceToHost));

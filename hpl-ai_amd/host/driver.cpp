@@ -31,6 +31,9 @@ static void usage()
         "   -r, --rdma        Accepted (no inter-process broadcast on one device).\n"
         "   -w, --warmup      One IR step only.\n"
         "   --checksum        Print the factor checksum line.\n"
+        "   --highammat       Higham's HPL-AI matrix as the reference builds it instead of\n"
+        "                     the random diagonally dominant matrix.\n"
+        "   --highammat-kappa Higham's matrix with cond(A, inf) = 1000 exactly.\n"
         "   --fmt <f>         Schur operands: f16 (default), f16t (per-tile 2^e), e4m3.\n"
         "   --outer <s>       Panels per trailing update (default 1).\n"
         "   --lookahead <0|1> Factor panel k+1 on a second stream (default 1).\n"
@@ -97,7 +100,8 @@ int main(int argc, char **argv)
     p.verbose = 1;
     p.profile = 1;   // per-phase times for the Timerdump file
     int jobid = 0;
-    bool checksum = false, warmup = false, lazy = false, rdma = false, tile = true;
+    bool checksum = false, warmup = false, lazy = false, rdma = false, tile = true, higham = false;
+    bool higham_kappa = false;
     int device = 0;
     for (int i = 4; i < argc; ++i) {
         const char *a = argv[i];
@@ -108,6 +112,8 @@ int main(int argc, char **argv)
         else if (!std::strcmp(a, "-r") || !std::strcmp(a, "--rdma")) rdma = true;
         else if (!std::strcmp(a, "-w") || !std::strcmp(a, "--warmup")) warmup = true;
         else if (!std::strcmp(a, "--checksum")) checksum = true;
+        else if (!std::strcmp(a, "--highammat")) higham = true;
+        else if (!std::strcmp(a, "--highammat-kappa")) higham = higham_kappa = true;
         else if (!std::strcmp(a, "--fmt")) {
             const char *f = next();
             if (!std::strcmp(f, "f16")) p.fmt = HPLAI_FMT_F16;
@@ -126,12 +132,14 @@ int main(int argc, char **argv)
     }
     if (warmup) p.maxit = 1;
     p.lazy = lazy ? 1 : 0;
+    p.matrix = higham_kappa ? HPLAI_MATRIX_HIGHAM_KAPPA : (higham ? HPLAI_MATRIX_HIGHAM : HPLAI_MATRIX_HPLAI);
+    p.cond = 1000.0;
 
     std::printf("jobid=%d\n", jobid);
     std::printf("n=%lld b=%d r=%d c=%d\n", n, b, 1, 1);
-    std::printf("%s %s %s %s nopack %s noskiplu ddmat\n", tile ? "tile" : "2dbc",
+    std::printf("%s %s %s %s nopack %s noskiplu %s\n", tile ? "tile" : "2dbc",
                 lazy ? "lazy" : "precomp", rdma ? "rdma" : "mpi", warmup ? "warmup" : "full",
-                checksum ? "checksum" : "nocheck");
+                checksum ? "checksum" : "nocheck", higham ? "hmat" : "ddmat");
     std::printf("device=%d P=%d fmt=%s outer=%d lookahead=%d\n", device, nrow,
                 p.fmt == HPLAI_FMT_F16 ? "f16" : (p.fmt == HPLAI_FMT_F16_TILE ? "f16t" : "e4m3"),
                 p.outer, p.lookahead);

@@ -16,6 +16,16 @@ DRIVER_PATH = os.path.join(os.path.dirname(_HERE), "bin", "driver.out")
 OK, EINVAL, EHIP, ENOMEM, ENOCONV, EPIVOT = 0, -1, -2, -3, -4, -5
 FMT_F16, FMT_F16_TILE, FMT_E4M3 = 0, 1, 2
 FMTS = {"f16": FMT_F16, "f16t": FMT_F16_TILE, "e4m3": FMT_E4M3, "fp8": FMT_E4M3}
+MATRIX_HPLAI, MATRIX_HIGHAM, MATRIX_HIGHAM_KAPPA = 0, 1, 2
+MATRICES = {"hplai": MATRIX_HPLAI, "higham": MATRIX_HIGHAM, "higham_kappa": MATRIX_HIGHAM_KAPPA}
+
+
+def higham_coeffs(n, cond=1000.0, rho=0.5):
+    """(alpha, beta) of Higham's HPL-AI matrix of order n with cond(A, inf) = cond
+    (host computation, no GPU needed)."""
+    a, b = _dbl(), _dbl()
+    check(lib().hplai_higham_coeffs(n, cond, rho, ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
 
 _i64, _u64, _int, _dbl = ctypes.c_int64, ctypes.c_uint64, ctypes.c_int, ctypes.c_double
 _vp = ctypes.c_void_p
@@ -30,7 +40,7 @@ class HplaiError(RuntimeError):
 class Params(ctypes.Structure):
     _fields_ = [("n", _i64), ("b", _int), ("seed", _u64), ("fmt", _int), ("outer", _int),
                 ("lookahead", _int), ("lazy", _int), ("maxit", _int), ("verbose", _int),
-                ("keep_factors", _int), ("profile", _int)]
+                ("keep_factors", _int), ("profile", _int), ("matrix", _int), ("cond", _dbl)]
 
 
 class Result(ctypes.Structure):
@@ -69,6 +79,9 @@ _SIGS = [
     ("hplai_schur_update", _int, [_vp, _int, _i64, _i64, _int, _int, _vp, _i64, _vp, _i64, _vp, _i64,
                                   _vp, _i64, _int, _vp, _i64, _vp]),
     ("hplai_otf_gemv_f64", _int, [_vp, _i64, _u64, _vp, _dbl, _vp, _vp, _vp]),
+    ("hplai_higham_coeffs", _int, [_i64, _dbl, _dbl, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]),
+    ("hplai_gen_higham_f32", _int, [_vp, _i64, _dbl, _dbl, _i64, _i64, _i64, _i64, _vp, _i64, _vp]),
+    ("hplai_higham_gemv_f64", _int, [_vp, _i64, _dbl, _dbl, _dbl, _vp, _vp, _vp]),
     ("hplai_trsv_lower_f32", _int, [_vp, _i64, _int, _vp, _i64, _vp, _vp]),
     ("hplai_trsv_upper_f32", _int, [_vp, _i64, _int, _vp, _i64, _vp, _vp]),
     ("hplai_norm_inf_f64", _int, [_vp, _i64, _vp, _vp, _vp]),
@@ -202,6 +215,15 @@ class Context:
                                        stream_ptr(stream)))
 
     # ---- IR ----
+    # ---- Higham's HPL-AI matrix ----
+    def gen_higham(self, n, alpha, beta, i0, j0, m, k, A, lda, stream=None):
+        check(lib().hplai_gen_higham_f32(self.handle, n, alpha, beta, i0, j0, m, k, ptr(A), lda,
+                                         stream_ptr(stream)))
+
+    def higham_gemv(self, n, alpha, beta, scale, x, y, stream=None):
+        check(lib().hplai_higham_gemv_f64(self.handle, n, alpha, beta, scale, ptr(x), ptr(y),
+                                          stream_ptr(stream)))
+
     def otf_gemv(self, n, seed, d, alpha, x, y, stream=None):
         check(lib().hplai_otf_gemv_f64(self.handle, n, seed, ptr(d), alpha, ptr(x), ptr(y),
                                        stream_ptr(stream)))
@@ -228,11 +250,15 @@ class Context:
 
     # ---- whole solve ----
     def solve(self, n, b, fmt="f16", outer=1, lookahead=True, lazy=False, seed=42, maxit=50,
-              verbose=False, keep_factors=False, profile=False):
+              verbose=False, keep_factors=False, profile=False, matrix="hplai", cond=0.0):
+        """matrix: "hplai" (the reference's LCG matrix), "higham" (--highammat as
+        the reference generates it) or "higham_kappa" (Fasi and Higham's signs,
+        cond(A, inf) = cond exactly); cond 0 = 1000."""
         p = Params(n=n, b=b, seed=seed, fmt=FMTS[fmt] if isinstance(fmt, str) else fmt, outer=outer,
                    lookahead=1 if lookahead else 0, lazy=1 if lazy else 0, maxit=maxit,
                    verbose=1 if verbose else 0, keep_factors=1 if keep_factors else 0,
-                   profile=1 if profile else 0)
+                   profile=1 if profile else 0, matrix=MATRICES[matrix] if isinstance(matrix, str) else matrix,
+                   cond=cond)
         r = Result()
         rc = lib().hplai_solve(self.handle, ctypes.byref(p), ctypes.byref(r))
         if rc not in (OK, ENOCONV):

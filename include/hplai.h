@@ -104,6 +104,24 @@ int hplai_schur_update(hplai_ctx ctx, int fmt, int64_t m, int64_t n, int nslices
                        const void *Uq, int64_t ldu, const int32_t *eU, int64_t ldeu, int tb,
                        float *C, int64_t ldc, void *stream);
 
+/* ---------------- Higham's HPL-AI matrix (--highammat) ---------------- */
+/* Replaces HMGen (highammgen.hpp:5-27) with higham_mat_comp_beta
+ * (higham_mat_impl.cpp:83-110): alpha = rho * beta with beta chosen so that
+ * A = L U (L unit lower with alpha below the diagonal, U unit upper with beta
+ * above it) has cond(A, inf) = cond.  The reference uses cond = 1000,
+ * rho = 0.5 (main.cpp:258).  Host-only; no context needed. */
+int hplai_higham_coeffs(int64_t n, double cond, double rho, double *alpha, double *beta);
+/* Replaces pmatgen(HMGen) (matgen.hpp:24-78): the FP32 block
+ * A(i0:i0+m, j0:j0+k) of the Higham matrix, column-major with lda.  alpha and
+ * beta are signed: (+alpha, +beta) of hplai_higham_coeffs for the reference's
+ * matrix, (-alpha, -beta) for the exact-cond one. */
+int hplai_gen_higham_f32(hplai_ctx ctx, int64_t n, double alpha, double beta, int64_t i0, int64_t j0,
+                         int64_t m, int64_t k, float *A, int64_t lda, void *stream);
+/* Replaces otf_gemv(HMGen) (panel_gemv.hpp:86, otf_gemv.cpp:133-314):
+ * y := y + scale * A x in FP64 with the Higham matrix regenerated. */
+int hplai_higham_gemv_f64(hplai_ctx ctx, int64_t n, double alpha, double beta, double scale,
+                          const double *x, double *y, void *stream);
+
 /* ---------------- iterative refinement (FP64) ---------------- */
 /* Replaces panel_gemv / otf_gemv / otf_gemv_kernel (panel_gemv.hpp:57-180,
  * otf_gemv.cpp:5-131): y := y + alpha * A x with A regenerated on the fly
@@ -194,7 +212,17 @@ typedef struct hplai_params {
     int      verbose;    /* print the reference's IR trace and !epoch lines        */
     int      keep_factors; /* leave A (factors) allocated for hplai_solution_*     */
     int      profile;    /* time every Schur-update launch with HIP events         */
+    int      matrix;     /* HPLAI_MATRIX_*: the input matrix family                */
+    double   cond;       /* Higham matrix: cond(A, inf); 0 = 1000 as in main.cpp:258 */
 } hplai_params;
+
+#define HPLAI_MATRIX_HPLAI        0   /* the reference's LCG matrix with diagonal row sums */
+#define HPLAI_MATRIX_HIGHAM       1   /* --highammat as the reference generates it (main.cpp:85,
+                                         258-276; pmatgen(HMGen) matgen.hpp:24-78): L U with
+                                         +alpha below and +beta above the unit diagonals */
+#define HPLAI_MATRIX_HIGHAM_KAPPA 2   /* no reference counterpart: Fasi and Higham's signs
+                                         (-alpha, -beta, higham_mat_impl.cpp:113-125), so that
+                                         cond(A, inf) = cond exactly */
 
 /* Phases of the reference's Timer (timer.hpp:58-83), same order and names;
  * the communication phases are 0 on one GPU. */
