@@ -111,6 +111,19 @@ constexpr int kF16Er = 15;
 
 __device__ __forceinline__ float ldexp_fast(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
 
+// Panel-packed layout of the solver's quantized operand panels: rows in
+// panels of 256, each panel's K bytes in stages of 64 bytes, and the
+// 256 x 64-byte block of one (panel, stage) contiguous (16 KB, row r at
+// r * 64), so one wave-wide 1 KB load covers 16 whole rows of a stage.  A
+// panel occupies the bytes of 256 rows of the row layout with the same row
+// length ld_b: a row offset that is a multiple of 256 moves the base by
+// row * ld_b in both layouts, a K offset that is a multiple of 64 by
+// 256 * kbyte here.
+__host__ __device__ __forceinline__ int64_t packed_off(int64_t row, int64_t kbyte, int64_t ld_b)
+{
+    return (row >> 8) * (ld_b << 8) + (kbyte >> 6) * 16384 + (row & 255) * 64 + (kbyte & 63);
+}
+
 // atomic max of a non-negative float, ordered as its uint32 bit pattern
 __device__ __forceinline__ void atomic_max_pos(float *p, float v)
 {

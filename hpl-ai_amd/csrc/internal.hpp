@@ -68,15 +68,18 @@ int launch_trsm_l(int64_t m, int b, const float *U, int64_t ldu, float *L, int64
 int launch_trsm_u(int b, int64_t n, const float *Lkk, int64_t ldl, float *U, int64_t ldu,
                   float *amax, int tb, const float *inv, hipStream_t s);
 // amax: per-tile maxima of |src| (nullable for FMT_F16 → exponents 0).
+// packed: dst in the panel-packed layout (packed_off), rows a multiple of 256 from its base.
 int launch_quantize(int fmt, int side, int64_t m, int64_t k, int tb, const float *src, int64_t lds,
-                    void *dst, int64_t ldd, const float *amax, int32_t *exps, hipStream_t s);
+                    void *dst, int64_t ldd, const float *amax, int32_t *exps, hipStream_t s,
+                    bool packed = false);
 int launch_tile_amax(int side, int64_t m, int64_t k, int tb, const float *src, int64_t lds,
                      float *amax, hipStream_t s);
 int launch_fill_f32(float *p, int64_t cnt, float v, hipStream_t s);
+// packed: Lq and Uq in the panel-packed layout (packed_off) from their bases.
 int launch_schur(int fmt, int64_t m, int64_t n, int nslices, int kslice, const void *Lq,
                  int64_t ldl, const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
                  const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, int num_cu,
-                 hipStream_t s);
+                 hipStream_t s, bool packed = false);
 // true when launch_schur runs the streamed 256 x 256 kernel for this shape
 bool schur_large_tiles(int fmt, int64_t m, int64_t n, int tb, int num_cu);
 

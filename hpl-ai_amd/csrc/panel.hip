@@ -536,7 +536,7 @@ __device__ __forceinline__ int fmt_exponent(const float *amax, int64_t tile)
                                  : tile_exponent(a, kF16MrFrac, kF16Er);
 }
 
-template <int FMT>
+template <int FMT, bool PACKED>
 __global__ __launch_bounds__(256) void quant_l_kernel(int64_t m, int64_t k, int tb,
                                                       const float *__restrict__ src, int64_t lds,
                                                       void *__restrict__ dst, int64_t ldd,
@@ -564,17 +564,19 @@ __global__ __launch_bounds__(256) void quant_l_kernel(int64_t m, int64_t k, int 
         for (int q = 0; q < 4; ++q) v[q] = ldexp_fast(T[kc + q][r], e);
         const int nk = k - (c0 + kc) < 4 ? static_cast<int>(k - (c0 + kc)) : 4;
         if (FMT == HPLAI_FMT_E4M3) {
-            uint8_t *d8 = static_cast<uint8_t *>(dst) + gr * ldd + c0 + kc;
+            uint8_t *d8 = static_cast<uint8_t *>(dst) + (PACKED ? packed_off(gr, c0 + kc, ldd) : gr * ldd + c0 + kc);
             uint32_t w = f32x4_to_e4m3x4(v[0], v[1], v[2], v[3]);
             for (int q = 0; q < nk; ++q) d8[q] = static_cast<uint8_t>(w >> (8 * q));
         } else {
-            uint16_t *d16 = static_cast<uint16_t *>(dst) + gr * ldd + c0 + kc;
+            uint16_t *d16 = PACKED ? reinterpret_cast<uint16_t *>(static_cast<uint8_t *>(dst) +
+                                                                packed_off(gr, 2 * (c0 + kc), 2 * ldd))
+                                   : static_cast<uint16_t *>(dst) + gr * ldd + c0 + kc;
             for (int q = 0; q < nk; ++q) d16[q] = f32_to_f16_bits(v[q]);
         }
     }
 }
 
-template <int FMT>
+template <int FMT, bool PACKED>
 __global__ __launch_bounds__(256) void quant_u_kernel(int64_t m, int64_t k, int tb,
                                                       const float *__restrict__ src, int64_t lds,
                                                       void *__restrict__ dst, int64_t ldd,
@@ -593,11 +595,12 @@ __global__ __launch_bounds__(256) void quant_u_kernel(int64_t m, int64_t k, int 
 #pragma unroll
     for (int q = 0; q < 4; ++q) v[q] = q < nk ? ldexp_fast(src[j * lds + r + q], e) : 0.0f;
     if (FMT == HPLAI_FMT_E4M3) {
-        uint8_t *d8 = static_cast<uint8_t *>(dst) + j * ldd + r;
+        uint8_t *d8 = static_cast<uint8_t *>(dst) + (PACKED ? packed_off(j, r, ldd) : j * ldd + r);
         uint32_t w = f32x4_to_e4m3x4(v[0], v[1], v[2], v[3]);
         for (int q = 0; q < nk; ++q) d8[q] = static_cast<uint8_t>(w >> (8 * q));
     } else {
-        uint16_t *d16 = static_cast<uint16_t *>(dst) + j * ldd + r;
+        uint16_t *d16 = PACKED ? reinterpret_cast<uint16_t *>(static_cast<uint8_t *>(dst) + packed_off(j, 2 * r, 2 * ldd))
+                               : static_cast<uint16_t *>(dst) + j * ldd + r;
         for (int q = 0; q < nk; ++q) d16[q] = f32_to_f16_bits(v[q]);
     }
 }
@@ -686,34 +689,42 @@ int launch_fill_f32(float *p, int64_t cnt, float v, hipStream_t s)
     return HPLAI_OK;
 }
 
-template <int FMT>
+template <int FMT, bool PACKED>
 static int quant_dispatch(int side, int64_t m, int64_t k, int tb, const float *src, int64_t lds,
                           void *dst, int64_t ldd, const float *amax, int32_t *exps, hipStream_t s)
 {
     if (side == 0) {
         dim3 grid(static_cast<unsigned>(ceil_div(m, 64)), static_cast<unsigned>(ceil_div(k, 64)));
-        hipLaunchKernelGGL(quant_l_kernel<FMT>, grid, dim3(256), 0, s, m, k, tb, src, lds, dst, ldd,
+        hipLaunchKernelGGL((quant_l_kernel<FMT, PACKED>), grid, dim3(256), 0, s, m, k, tb, src, lds, dst, ldd,
                            amax, exps);
     } else {
         const int64_t thr = m * ceil_div(k, 4);
-        hipLaunchKernelGGL(quant_u_kernel<FMT>, dim3(static_cast<unsigned>(ceil_div(thr, 256))),
+        hipLaunchKernelGGL((quant_u_kernel<FMT, PACKED>), dim3(static_cast<unsigned>(ceil_div(thr, 256))),
                            dim3(256), 0, s, m, k, tb, src, lds, dst, ldd, amax, exps);
     }
     HPLAI_HIP(hipGetLastError());
     return HPLAI_OK;
 }
 
+template <int FMT>
+static int quant_dispatch(bool packed, int side, int64_t m, int64_t k, int tb, const float *src, int64_t lds,
+                          void *dst, int64_t ldd, const float *amax, int32_t *exps, hipStream_t s)
+{
+    return packed ? quant_dispatch<FMT, true>(side, m, k, tb, src, lds, dst, ldd, amax, exps, s)
+                  : quant_dispatch<FMT, false>(side, m, k, tb, src, lds, dst, ldd, amax, exps, s);
+}
+
 int launch_quantize(int fmt, int side, int64_t m, int64_t k, int tb, const float *src, int64_t lds,
-                    void *dst, int64_t ldd, const float *amax, int32_t *exps, hipStream_t s)
+                    void *dst, int64_t ldd, const float *amax, int32_t *exps, hipStream_t s, bool packed)
 {
     if (m <= 0 || k <= 0) return HPLAI_OK;
     switch (fmt) {
     case HPLAI_FMT_F16:
-        return quant_dispatch<HPLAI_FMT_F16>(side, m, k, tb, src, lds, dst, ldd, amax, exps, s);
+        return quant_dispatch<HPLAI_FMT_F16>(packed, side, m, k, tb, src, lds, dst, ldd, amax, exps, s);
     case HPLAI_FMT_F16_TILE:
-        return quant_dispatch<HPLAI_FMT_F16_TILE>(side, m, k, tb, src, lds, dst, ldd, amax, exps, s);
+        return quant_dispatch<HPLAI_FMT_F16_TILE>(packed, side, m, k, tb, src, lds, dst, ldd, amax, exps, s);
     case HPLAI_FMT_E4M3:
-        return quant_dispatch<HPLAI_FMT_E4M3>(side, m, k, tb, src, lds, dst, ldd, amax, exps, s);
+        return quant_dispatch<HPLAI_FMT_E4M3>(packed, side, m, k, tb, src, lds, dst, ldd, amax, exps, s);
     default:
         set_error("quantize: unknown format %d", fmt);
         return HPLAI_EINVAL;

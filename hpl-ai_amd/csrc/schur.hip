@@ -108,7 +108,7 @@ __device__ __forceinline__ void epilogue_block_row(float *__restrict__ C, int64_
     }
 }
 
-template <int FMT, int BM, int BN, bool VEC_C>
+template <int FMT, int BM, int BN, bool VEC_C, bool PACKED>
 __global__ __launch_bounds__(kThreads) void schur_kernel(
     int64_t M, int64_t N, int nslices, int kslice, const uint8_t *__restrict__ Lq, int64_t ldl_b,
     const int32_t *__restrict__ eL, int64_t ldel, const uint8_t *__restrict__ Uq, int64_t ldu_b,
@@ -154,14 +154,18 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
             const int c = tid + kThreads * q;
             const int row = c >> 3, part = c & 7;
             const int64_t gm = m0 + row;
-            ra[q] = *reinterpret_cast<const intx4_t *>(Lq + (gm < M ? gm : M - 1) * ldl_b + kb + part * 16);
+            const int64_t rm = gm < M ? gm : M - 1;
+            ra[q] = *reinterpret_cast<const intx4_t *>(
+                Lq + (PACKED ? packed_off(rm, kb + part * 16, ldl_b) : rm * ldl_b + kb + part * 16));
         }
 #pragma unroll
         for (int q = 0; q < QB; ++q) {
             const int c = tid + kThreads * q;
             const int row = c >> 3, part = c & 7;
             const int64_t gn = n0 + row;
-            rb[q] = *reinterpret_cast<const intx4_t *>(Uq + (gn < N ? gn : N - 1) * ldu_b + kb + part * 16);
+            const int64_t rn = gn < N ? gn : N - 1;
+            rb[q] = *reinterpret_cast<const intx4_t *>(
+                Uq + (PACKED ? packed_off(rn, kb + part * 16, ldu_b) : rn * ldu_b + kb + part * 16));
         }
     };
     auto sstore = [&](int buf) {
@@ -311,7 +315,7 @@ constexpr int kDmaStages = 4;
 constexpr int kDmaStage = 2 * kDmaTile * kDmaStageBytes;           // 32 KB
 constexpr int kDmaLoadsPerStage = 2 * kDmaTile * kDmaStageBytes / (kThreads * 16);   // 8 per thread
 
-template <int FMT, bool VEC_C>
+template <int FMT, bool VEC_C, bool PACKED>
 __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
     int64_t M, int64_t N, int nslices, int kslice, const uint8_t *__restrict__ Lq, int64_t ldl_b,
     const int32_t *__restrict__ eL, int64_t ldel, const uint8_t *__restrict__ Uq, int64_t ldu_b,
@@ -352,8 +356,8 @@ __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
             const int row = (4 * q + wave) * 16 + (lane >> 2);
             const int64_t gm = m0 + row < M ? m0 + row : M - 1;
             const int64_t gn = n0 + row < N ? n0 + row : N - 1;
-            srcA[q] = Lq + gm * ldl_b + chunk * 16;
-            srcB[q] = Uq + gn * ldu_b + chunk * 16;
+            srcA[q] = Lq + (PACKED ? packed_off(gm, chunk * 16, ldl_b) : gm * ldl_b + chunk * 16);
+            srcB[q] = Uq + (PACKED ? packed_off(gn, chunk * 16, ldu_b) : gn * ldu_b + chunk * 16);
         }
     }
     const int nstages = static_cast<int>(static_cast<int64_t>(nslices) * kslice * F::es / SB);
@@ -361,7 +365,8 @@ __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
 
     // stage `st` of K into LDS buffer `buf`
     auto issue = [&](int st, int buf) {
-        const int64_t kb = static_cast<int64_t>(st) * SB;
+        // next K stage: SB bytes on in the row layout, one 16 KB block on when packed
+        const int64_t kb = static_cast<int64_t>(st) * (PACKED ? 256 * SB : SB);
         uint8_t *dst = S + buf * kDmaStage + wave * 1024;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -524,7 +529,7 @@ __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
                                      acc[i]);
 }
 
-template <int FMT>
+template <int FMT, bool PACKED>
 static int schur_dma_launch(int64_t m, int64_t n, int nslices, int kslice, const void *Lq, int64_t ldl,
                             const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
                             const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, hipStream_t s)
@@ -541,18 +546,18 @@ static int schur_dma_launch(int64_t m, int64_t n, int nslices, int kslice, const
     const uint8_t *L8 = static_cast<const uint8_t *>(Lq);
     const uint8_t *U8 = static_cast<const uint8_t *>(Uq);
     if (vec)
-        hipLaunchKernelGGL((schur_dma_kernel<FMT, true>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
+        hipLaunchKernelGGL((schur_dma_kernel<FMT, true, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
                            L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
                            static_cast<int>(tiles_m), static_cast<int>(tiles_n));
     else
-        hipLaunchKernelGGL((schur_dma_kernel<FMT, false>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
+        hipLaunchKernelGGL((schur_dma_kernel<FMT, false, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
                            L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
                            static_cast<int>(tiles_m), static_cast<int>(tiles_n));
     HPLAI_HIP(hipGetLastError());
     return HPLAI_OK;
 }
 
-template <int FMT, int BM, int BN>
+template <int FMT, int BM, int BN, bool PACKED>
 static int schur_launch(int64_t m, int64_t n, int nslices, int kslice, const void *Lq, int64_t ldl,
                         const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
                         const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, hipStream_t s)
@@ -569,11 +574,11 @@ static int schur_launch(int64_t m, int64_t n, int nslices, int kslice, const voi
     const uint8_t *L8 = static_cast<const uint8_t *>(Lq);
     const uint8_t *U8 = static_cast<const uint8_t *>(Uq);
     if (vec)
-        hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, true>), grid, dim3(kThreads), 0, s, m, n, nslices,
+        hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, true, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices,
                            kslice, L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
                            static_cast<int>(tiles_m), static_cast<int>(tiles_n));
     else
-        hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, false>), grid, dim3(kThreads), 0, s, m, n, nslices,
+        hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, false, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices,
                            kslice, L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
                            static_cast<int>(tiles_m), static_cast<int>(tiles_n));
     HPLAI_HIP(hipGetLastError());
@@ -593,7 +598,7 @@ bool schur_large_tiles(int fmt, int64_t m, int64_t n, int tb, int num_cu)
     return ceil_div(m, kDmaTile) * ceil_div(n, kDmaTile) >= 2 * num_cu;
 }
 
-template <int FMT>
+template <int FMT, bool PACKED>
 static int schur_dispatch(int64_t m, int64_t n, int nslices, int kslice, const void *Lq, int64_t ldl,
                           const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
                           const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, int num_cu,
@@ -601,10 +606,10 @@ static int schur_dispatch(int64_t m, int64_t n, int nslices, int kslice, const v
 {
     if constexpr (FMT != HPLAI_FMT_F16_TILE) {
         if (schur_large_tiles(FMT, m, n, tb, num_cu))
-            return schur_dma_launch<FMT>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu,
+            return schur_dma_launch<FMT, PACKED>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu,
                                          tb, C, ldc, s);
     }
-    return schur_launch<FMT, 128, 128>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu, tb,
+    return schur_launch<FMT, 128, 128, PACKED>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu, tb,
                                        C, ldc, s);
 }
 
@@ -612,7 +617,7 @@ static int schur_dispatch(int64_t m, int64_t n, int nslices, int kslice, const v
 int launch_schur(int fmt, int64_t m, int64_t n, int nslices, int kslice, const void *Lq,
                  int64_t ldl, const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
                  const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, int num_cu,
-                 hipStream_t s)
+                 hipStream_t s, bool packed)
 {
     if (m <= 0 || n <= 0 || nslices <= 0) return HPLAI_OK;
     if (num_cu <= 0) num_cu = 256;
@@ -634,16 +639,25 @@ int launch_schur(int fmt, int64_t m, int64_t n, int nslices, int kslice, const v
         HPLAI_CHECK(eL != nullptr && eU != nullptr, "schur_update: exponent arrays required");
         HPLAI_CHECK(tb > 0 && tb % 128 == 0, "schur_update: tile %d must be a multiple of 128", tb);
     }
+    if (packed)
+        HPLAI_CHECK((ldl * es) % 64 == 0 && (ldu * es) % 64 == 0,
+                    "schur_update: packed operands need rows of whole 64-byte stages");
     switch (fmt) {
     case HPLAI_FMT_F16:
-        return schur_dispatch<HPLAI_FMT_F16>(m, n, nslices, kslice, Lq, ldl, nullptr, 0, Uq, ldu,
-                                             nullptr, 0, 0, C, ldc, num_cu, s);
+        return packed ? schur_dispatch<HPLAI_FMT_F16, true>(m, n, nslices, kslice, Lq, ldl, nullptr, 0, Uq, ldu,
+                                                            nullptr, 0, 0, C, ldc, num_cu, s)
+                      : schur_dispatch<HPLAI_FMT_F16, false>(m, n, nslices, kslice, Lq, ldl, nullptr, 0, Uq, ldu,
+                                                             nullptr, 0, 0, C, ldc, num_cu, s);
     case HPLAI_FMT_F16_TILE:
-        return schur_dispatch<HPLAI_FMT_F16_TILE>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu,
-                                                  eU, ldeu, tb, C, ldc, num_cu, s);
+        return packed ? schur_dispatch<HPLAI_FMT_F16_TILE, true>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq,
+                                                                 ldu, eU, ldeu, tb, C, ldc, num_cu, s)
+                      : schur_dispatch<HPLAI_FMT_F16_TILE, false>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq,
+                                                                  ldu, eU, ldeu, tb, C, ldc, num_cu, s);
     default:
-        return schur_dispatch<HPLAI_FMT_E4M3>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU,
-                                              ldeu, tb, C, ldc, num_cu, s);
+        return packed ? schur_dispatch<HPLAI_FMT_E4M3, true>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu,
+                                                             eU, ldeu, tb, C, ldc, num_cu, s)
+                      : schur_dispatch<HPLAI_FMT_E4M3, false>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu,
+                                                              eU, ldeu, tb, C, ldc, num_cu, s);
     }
 }
 

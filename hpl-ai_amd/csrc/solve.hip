@@ -240,6 +240,7 @@ struct LU {
     int b, outer, fmt, kpad, es;
     bool lazy;
     bool higham = false;          // Higham's matrix instead of the LCG matrix
+    bool packed = false;          // Lq / Uq in the panel-packed layout (b % 256 == 0)
     double h_alpha = 0.0, h_beta = 0.0;
     uint64_t seed;
     int64_t ntiles;
@@ -252,14 +253,15 @@ struct LU {
 
     bool tiled() const { return fmt != HPLAI_FMT_F16; }
     float *a(int64_t i, int64_t j) const { return B->A + j * lda + i; }
-    uint8_t *lq(int buf, int64_t row, int t) const
+    // slice t of row (column) `row` of the quantized panels; packed: row is a
+    // multiple of 256 and the K offset moves whole 16 KB stage blocks
+    int64_t qoff(int64_t row, int t) const
     {
-        return static_cast<uint8_t *>(B->Lq[buf]) + (row * ldq + static_cast<int64_t>(t) * kpad) * es;
+        const int64_t kb = static_cast<int64_t>(t) * kpad * es;
+        return row * ldq * es + (packed ? 256 * kb : kb);
     }
-    uint8_t *uq(int buf, int64_t col, int t) const
-    {
-        return static_cast<uint8_t *>(B->Uq[buf]) + (col * ldq + static_cast<int64_t>(t) * kpad) * es;
-    }
+    uint8_t *lq(int buf, int64_t row, int t) const { return static_cast<uint8_t *>(B->Lq[buf]) + qoff(row, t); }
+    uint8_t *uq(int buf, int64_t col, int t) const { return static_cast<uint8_t *>(B->Uq[buf]) + qoff(col, t); }
     int32_t *el(int buf, int t, int64_t row) const
     {
         return tiled() ? B->eL[buf] + t * ntiles + row / b : nullptr;
@@ -288,7 +290,7 @@ struct LU {
                      schur_large_tiles(fmt, m, nn, b, num_cu), nsl == 1 && outer > 1);
         const int rc = launch_schur(fmt, m, nn, nsl, kpad, lq(buf, row0, t0), ldq, el(buf, t0, row0),
                                     ntiles, uq(buf, col0, t0), ldq, eu(buf, t0, col0), ntiles, b,
-                                    a(row0, col0), lda, num_cu, s);
+                                    a(row0, col0), lda, num_cu, s, packed);
         timer->end(s);
         return rc;
     }
@@ -373,7 +375,7 @@ struct LU {
                 if (rc == HPLAI_OK)
                     rc = timed(HPLAI_PHASE_CONV_L, static_cast<double>(mb) * bd, q, [&] {
                         return launch_quantize(fmt, 0, mb, b, b, a(rb0, c), lda, lq(buf, rb0, t), ldq,
-                                               aml(buf, t, rb0), el(buf, t, rb0), q);
+                                               aml(buf, t, rb0), el(buf, t, rb0), q, packed);
                     });
                 if (rc) break;
             }
@@ -387,7 +389,7 @@ struct LU {
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_CONV_L, static_cast<double>(mrow) * bd, s, [&] {
                     return launch_quantize(fmt, 0, mrow, b, b, a(c + b, c), lda, lq(buf, c + b, t), ldq,
-                                           aml(buf, t, c + b), el(buf, t, c + b), s);
+                                           aml(buf, t, c + b), el(buf, t, c + b), s, packed);
                 });
             const int64_t ncols = rb0 - (c + b);
             if (rc) break;
@@ -400,7 +402,7 @@ struct LU {
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_CONV_R, static_cast<double>(ncols) * bd, s, [&] {
                     return launch_quantize(fmt, 1, ncols, b, b, a(c, c + b), lda, uq(buf, c + b, t), ldq,
-                                           amu(buf, t, c + b), eu(buf, t, c + b), s);
+                                           amu(buf, t, c + b), eu(buf, t, c + b), s, packed);
                 });
             if (rc) break;
             if (mb > 0) HPLAI_HIP(hipEventRecord(ev_u, s));
@@ -436,7 +438,7 @@ struct LU {
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_CONV_R, static_cast<double>(ncols) * bd, s, [&] {
                     return launch_quantize(fmt, 1, ncols, b, b, a(c, col0), lda, uq(buf, col0, t), ldq,
-                                           amu(buf, t, col0), eu(buf, t, col0), s);
+                                           amu(buf, t, col0), eu(buf, t, col0), s, packed);
                 });
         }
         return rc;
@@ -507,6 +509,9 @@ void init_lu(LU &lu, hplai_ctx ctx, int64_t n, int64_t lda, int b, int fmt, int 
     lu.es = es;
     lu.kpad = (b + bk - 1) / bk * bk;
     lu.ldq = static_cast<int64_t>(outer) * lu.kpad;
+    // every panel row / column offset is a multiple of b, so with b % 256 == 0
+    // the operands can use the panel-packed layout (whole 1 KB wave loads)
+    lu.packed = b % 256 == 0;
     lu.ntiles = n / b;
     lu.lazy = lazy;
     lu.seed = seed;
