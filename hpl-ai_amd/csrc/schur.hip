@@ -315,7 +315,10 @@ constexpr int kDmaStages = 4;
 constexpr int kDmaStage = 2 * kDmaTile * kDmaStageBytes;           // 32 KB
 constexpr int kDmaLoadsPerStage = 2 * kDmaTile * kDmaStageBytes / (kThreads * 16);   // 8 per thread
 
-template <int FMT, bool VEC_C, bool PACKED>
+// EPI: 0 = element-wise C access (unaligned C), 1 = float4 per accumulator
+// row block with ragged edges, 2 = whole tiles only (M, N multiples of 256),
+// C transposed through LDS for coalesced access.
+template <int FMT, int EPI, bool PACKED>
 __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
     int64_t M, int64_t N, int nslices, int kslice, const uint8_t *__restrict__ Lq, int64_t ldl_b,
     const int32_t *__restrict__ eL, int64_t ldel, const uint8_t *__restrict__ Uq, int64_t ldu_b,
@@ -521,12 +524,55 @@ __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
     // drain the clamped tail loads before the workgroup can exit
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 
-    // ---- epilogue: C -= acc, one accumulator row block (16 float4 per
-    // lane) at a time with all its loads in flight together ----
+    // ---- epilogue: C -= acc ----
+    if constexpr (EPI == 2) {
+        // through LDS, one 32 x 128 accumulator block row per wave at a time:
+        // the MFMA layout gives each lane 4 rows of one column, so a direct
+        // float4 access touches 32 columns per instruction with 32 bytes each;
+        // transposed in LDS, 8 lanes cover one column's 128 bytes and an
+        // instruction reads or writes 8 whole columns
+        __syncthreads();   // every wave is done with the operand stages
+        constexpr int TS = 36;   // LDS column stride (floats), 16-byte aligned
+        float *T = reinterpret_cast<float *>(S) + wave * (BN / 2) * TS;
+        const int cl = lane >> 3, rq = (lane & 7) * 4;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-        epilogue_block_row<4, VEC_C>(C, ldc, M, N, m0 + wm * (BM / 2) + i * 32, n0 + wn * (BN / 2), lrow, lgrp,
-                                     acc[i]);
+        for (int i = 0; i < 4; ++i) {
+            const int64_t row0 = m0 + wm * (BM / 2) + i * 32;
+            float *Cb = C + (n0 + wn * (BN / 2)) * ldc + row0 + rq;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float4 o = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2],
+                                      acc[i][j][4 * q + 3]};
+                    *reinterpret_cast<float4 *>(T + (j * 32 + lrow) * TS + 8 * q + 4 * lgrp) = o;
+                }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                float4 cv[8];
+#pragma unroll
+                for (int p = 0; p < 8; ++p)
+                    cv[p] = *reinterpret_cast<const float4 *>(Cb + static_cast<int64_t>((8 * h + p) * 8 + cl) * ldc);
+#pragma unroll
+                for (int p = 0; p < 8; ++p) {
+                    const float4 a = *reinterpret_cast<const float4 *>(T + ((8 * h + p) * 8 + cl) * TS + rq);
+                    float4 c4 = cv[p];
+                    c4.x -= a.x;
+                    c4.y -= a.y;
+                    c4.z -= a.z;
+                    c4.w -= a.w;
+                    *reinterpret_cast<float4 *>(Cb + static_cast<int64_t>((8 * h + p) * 8 + cl) * ldc) = c4;
+                }
+            }
+        }
+    } else {
+        // one accumulator row block (16 float4 per lane) at a time with all its
+        // loads in flight together; ragged edges element by element
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            epilogue_block_row<4, EPI == 1>(C, ldc, M, N, m0 + wm * (BM / 2) + i * 32, n0 + wn * (BN / 2), lrow,
+                                            lgrp, acc[i]);
+    }
 }
 
 template <int FMT, bool PACKED>
@@ -545,12 +591,17 @@ static int schur_dma_launch(int64_t m, int64_t n, int nslices, int kslice, const
     const dim3 grid(static_cast<unsigned>(tiles_m * tiles_n));
     const uint8_t *L8 = static_cast<const uint8_t *>(Lq);
     const uint8_t *U8 = static_cast<const uint8_t *>(Uq);
-    if (vec)
-        hipLaunchKernelGGL((schur_dma_kernel<FMT, true, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
+    const bool whole = vec && m % kDmaTile == 0 && n % kDmaTile == 0;
+    if (whole)
+        hipLaunchKernelGGL((schur_dma_kernel<FMT, 2, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
+                           L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
+                           static_cast<int>(tiles_m), static_cast<int>(tiles_n));
+    else if (vec)
+        hipLaunchKernelGGL((schur_dma_kernel<FMT, 1, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
                            L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
                            static_cast<int>(tiles_m), static_cast<int>(tiles_n));
     else
-        hipLaunchKernelGGL((schur_dma_kernel<FMT, false, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
+        hipLaunchKernelGGL((schur_dma_kernel<FMT, 0, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
                            L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
                            static_cast<int>(tiles_m), static_cast<int>(tiles_n));
     HPLAI_HIP(hipGetLastError());

@@ -201,7 +201,9 @@ def test_quantize_bit_exact(ctx, oracle, fmt, side):
                                             ("e4m3", 1000, 520, 1, 128),
                                             # >= 512 tiles of 256 x 256: the streamed large-tile kernel,
                                             # with ragged edges in both dimensions
-                                            ("f16", 5990, 5900, 2, 128), ("e4m3", 6000, 5950, 2, 256)])
+                                            ("f16", 5990, 5900, 2, 128), ("e4m3", 6000, 5950, 2, 256),
+                                            # whole tiles only: C transposed through LDS in the epilogue
+                                            ("f16", 6144, 6400, 2, 256), ("e4m3", 6400, 6144, 2, 256)])
 def test_schur_update_vs_fp64(ctx, oracle, fmt, m, n, nsl, tb):
     import hplai
     import torch
