@@ -566,12 +566,22 @@ __global__ __launch_bounds__(256) void quant_l_kernel(int64_t m, int64_t k, int 
         if (FMT == HPLAI_FMT_E4M3) {
             uint8_t *d8 = static_cast<uint8_t *>(dst) + (PACKED ? packed_off(gr, c0 + kc, ldd) : gr * ldd + c0 + kc);
             uint32_t w = f32x4_to_e4m3x4(v[0], v[1], v[2], v[3]);
-            for (int q = 0; q < nk; ++q) d8[q] = static_cast<uint8_t>(w >> (8 * q));
+            if (nk == 4 && (reinterpret_cast<uintptr_t>(d8) & 3) == 0)
+                *reinterpret_cast<uint32_t *>(d8) = w;   // one 4-byte store
+            else
+                for (int q = 0; q < nk; ++q) d8[q] = static_cast<uint8_t>(w >> (8 * q));
         } else {
             uint16_t *d16 = PACKED ? reinterpret_cast<uint16_t *>(static_cast<uint8_t *>(dst) +
                                                                 packed_off(gr, 2 * (c0 + kc), 2 * ldd))
                                    : static_cast<uint16_t *>(dst) + gr * ldd + c0 + kc;
-            for (int q = 0; q < nk; ++q) d16[q] = f32_to_f16_bits(v[q]);
+            if (nk == 4 && (reinterpret_cast<uintptr_t>(d16) & 7) == 0) {
+                uint2 w;
+                w.x = f32_to_f16_bits(v[0]) | (static_cast<uint32_t>(f32_to_f16_bits(v[1])) << 16);
+                w.y = f32_to_f16_bits(v[2]) | (static_cast<uint32_t>(f32_to_f16_bits(v[3])) << 16);
+                *reinterpret_cast<uint2 *>(d16) = w;     // one 8-byte store
+            } else {
+                for (int q = 0; q < nk; ++q) d16[q] = f32_to_f16_bits(v[q]);
+            }
         }
     }
 }
@@ -597,11 +607,21 @@ __global__ __launch_bounds__(256) void quant_u_kernel(int64_t m, int64_t k, int 
     if (FMT == HPLAI_FMT_E4M3) {
         uint8_t *d8 = static_cast<uint8_t *>(dst) + (PACKED ? packed_off(j, r, ldd) : j * ldd + r);
         uint32_t w = f32x4_to_e4m3x4(v[0], v[1], v[2], v[3]);
-        for (int q = 0; q < nk; ++q) d8[q] = static_cast<uint8_t>(w >> (8 * q));
+        if (nk == 4 && (reinterpret_cast<uintptr_t>(d8) & 3) == 0)
+            *reinterpret_cast<uint32_t *>(d8) = w;
+        else
+            for (int q = 0; q < nk; ++q) d8[q] = static_cast<uint8_t>(w >> (8 * q));
     } else {
         uint16_t *d16 = PACKED ? reinterpret_cast<uint16_t *>(static_cast<uint8_t *>(dst) + packed_off(j, 2 * r, 2 * ldd))
                                : static_cast<uint16_t *>(dst) + j * ldd + r;
-        for (int q = 0; q < nk; ++q) d16[q] = f32_to_f16_bits(v[q]);
+        if (nk == 4 && (reinterpret_cast<uintptr_t>(d16) & 7) == 0) {
+            uint2 w;
+            w.x = f32_to_f16_bits(v[0]) | (static_cast<uint32_t>(f32_to_f16_bits(v[1])) << 16);
+            w.y = f32_to_f16_bits(v[2]) | (static_cast<uint32_t>(f32_to_f16_bits(v[3])) << 16);
+            *reinterpret_cast<uint2 *>(d16) = w;
+        } else {
+            for (int q = 0; q < nk; ++q) d16[q] = f32_to_f16_bits(v[q]);
+        }
     }
 }
 

@@ -140,6 +140,13 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
     const int lrow = lane & 31, lgrp = lane >> 5;
 
     // ---- global -> register staging: QA / QB x 16 B per thread ----
+    // chunk c of a stage (128 bytes of K for every row): row layout, 8 lanes
+    // per row's 128 contiguous bytes; packed, the stage is two 64-byte blocks
+    // of 16 KB-strided stage blocks, so 4 lanes per row-half and consecutive
+    // lanes on consecutive rows keep every wave load on one contiguous range
+    static_assert(BM == BN, "one chunk map serves both operand tiles");
+    auto chunk_row = [&](int c) { return PACKED ? (c & (BM * 4 - 1)) >> 2 : c >> 3; };
+    auto chunk_part = [&](int c) { return PACKED ? ((c / (BM * 4)) << 2) | (c & 3) : c & 7; };
     intx4_t ra[QA], rb[QB];
     const int64_t kbytes_total = static_cast<int64_t>(nslices) * kslice * F::es;
     const int nstages = static_cast<int>(kbytes_total / kRowBytes);
@@ -152,7 +159,7 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
 #pragma unroll
         for (int q = 0; q < QA; ++q) {
             const int c = tid + kThreads * q;
-            const int row = c >> 3, part = c & 7;
+            const int row = chunk_row(c), part = chunk_part(c);
             const int64_t gm = m0 + row;
             const int64_t rm = gm < M ? gm : M - 1;
             ra[q] = *reinterpret_cast<const intx4_t *>(
@@ -161,7 +168,7 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
 #pragma unroll
         for (int q = 0; q < QB; ++q) {
             const int c = tid + kThreads * q;
-            const int row = c >> 3, part = c & 7;
+            const int row = chunk_row(c), part = chunk_part(c);
             const int64_t gn = n0 + row;
             const int64_t rn = gn < N ? gn : N - 1;
             rb[q] = *reinterpret_cast<const intx4_t *>(
@@ -172,14 +179,12 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
 #pragma unroll
         for (int q = 0; q < QA; ++q) {
             const int c = tid + kThreads * q;
-            const int row = c >> 3, part = c & 7;
-            *reinterpret_cast<intx4_t *>(&As[buf][row * kLdsRow + part * 16]) = ra[q];
+            *reinterpret_cast<intx4_t *>(&As[buf][chunk_row(c) * kLdsRow + chunk_part(c) * 16]) = ra[q];
         }
 #pragma unroll
         for (int q = 0; q < QB; ++q) {
             const int c = tid + kThreads * q;
-            const int row = c >> 3, part = c & 7;
-            *reinterpret_cast<intx4_t *>(&Bs[buf][row * kLdsRow + part * 16]) = rb[q];
+            *reinterpret_cast<intx4_t *>(&Bs[buf][chunk_row(c) * kLdsRow + chunk_part(c) * 16]) = rb[q];
         }
     };
 
