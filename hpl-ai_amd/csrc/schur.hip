@@ -108,7 +108,52 @@ __device__ __forceinline__ void epilogue_block_row(float *__restrict__ C, int64_
     }
 }
 
-template <int FMT, int BM, int BN, bool VEC_C, bool PACKED>
+// The same for a block row inside C (whole tile, 16-byte aligned C): the
+// MFMA layout gives each lane 4 rows of one column, so a direct float4 access
+// touches 32 columns per instruction with 32 bytes each; transposed through
+// the LDS scratch T (32 JB columns x 36 floats, private to the wave), 8 lanes
+// cover one column's 128 bytes and each instruction 8 whole columns.
+constexpr int kEpiTS = 36;   // LDS column stride in floats (16-byte aligned)
+
+template <int JB>
+__device__ __forceinline__ void epilogue_block_row_lds(float *__restrict__ C, int64_t ldc, int64_t row0,
+                                                       int64_t col0, int lane, float *T,
+                                                       const floatx16_t (&acc)[JB])
+{
+    const int lrow = lane & 31, lgrp = lane >> 5;
+    const int cl = lane >> 3, rq = (lane & 7) * 4;
+#pragma unroll
+    for (int j = 0; j < JB; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 o = {acc[j][4 * q], acc[j][4 * q + 1], acc[j][4 * q + 2], acc[j][4 * q + 3]};
+            *reinterpret_cast<float4 *>(T + (j * 32 + lrow) * kEpiTS + 8 * q + 4 * lgrp) = o;
+        }
+    float *Cb = C + col0 * ldc + row0 + rq;
+    constexpr int PASSES = JB * 32 / 8;
+#pragma unroll
+    for (int h = 0; h < PASSES / 8; ++h) {
+        float4 cv[8];
+#pragma unroll
+        for (int p = 0; p < 8; ++p)
+            cv[p] = *reinterpret_cast<const float4 *>(Cb + static_cast<int64_t>((8 * h + p) * 8 + cl) * ldc);
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            const float4 a = *reinterpret_cast<const float4 *>(T + ((8 * h + p) * 8 + cl) * kEpiTS + rq);
+            float4 c4 = cv[p];
+            c4.x -= a.x;
+            c4.y -= a.y;
+            c4.z -= a.z;
+            c4.w -= a.w;
+            *reinterpret_cast<float4 *>(Cb + static_cast<int64_t>((8 * h + p) * 8 + cl) * ldc) = c4;
+        }
+    }
+}
+
+// EPI: 0 = element-wise C access (unaligned C), 1 = float4 per accumulator
+// row block with ragged edges, 2 = whole tiles only (M, N multiples of the
+// tile), C transposed through LDS (epilogue_block_row_lds).
+template <int FMT, int BM, int BN, int EPI, bool PACKED>
 __global__ __launch_bounds__(kThreads) void schur_kernel(
     int64_t M, int64_t N, int nslices, int kslice, const uint8_t *__restrict__ Lq, int64_t ldl_b,
     const int32_t *__restrict__ eL, int64_t ldel, const uint8_t *__restrict__ Uq, int64_t ldu_b,
@@ -296,10 +341,19 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
 #pragma unroll
                 for (int r = 0; r < 16; ++r) acc[i][j][r] = ldexp_fast(acc[i][j][r], -e_cur);
     }
+    if constexpr (EPI == 2) {
+        static_assert(4 * (BN / 2) * kEpiTS * 4 <= 2 * BM * kLdsRow, "epilogue scratch fits the A stages");
+        __syncthreads();   // every wave is done with the operand stages
+        float *T = reinterpret_cast<float *>(&As[0][0]) + wave * (BN / 2) * kEpiTS;
 #pragma unroll
-    for (int i = 0; i < IB; ++i)
-        epilogue_block_row<JB, VEC_C>(C, ldc, M, N, m0 + wm * (BM / 2) + i * 32, n0 + wn * (BN / 2), lrow, lgrp,
-                                      acc[i]);
+        for (int i = 0; i < IB; ++i)
+            epilogue_block_row_lds<JB>(C, ldc, m0 + wm * (BM / 2) + i * 32, n0 + wn * (BN / 2), lane, T, acc[i]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < IB; ++i)
+            epilogue_block_row<JB, EPI == 1>(C, ldc, M, N, m0 + wm * (BM / 2) + i * 32, n0 + wn * (BN / 2), lrow,
+                                             lgrp, acc[i]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -531,45 +585,11 @@ __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
 
     // ---- epilogue: C -= acc ----
     if constexpr (EPI == 2) {
-        // through LDS, one 32 x 128 accumulator block row per wave at a time:
-        // the MFMA layout gives each lane 4 rows of one column, so a direct
-        // float4 access touches 32 columns per instruction with 32 bytes each;
-        // transposed in LDS, 8 lanes cover one column's 128 bytes and an
-        // instruction reads or writes 8 whole columns
         __syncthreads();   // every wave is done with the operand stages
-        constexpr int TS = 36;   // LDS column stride (floats), 16-byte aligned
-        float *T = reinterpret_cast<float *>(S) + wave * (BN / 2) * TS;
-        const int cl = lane >> 3, rq = (lane & 7) * 4;
+        float *T = reinterpret_cast<float *>(S) + wave * (BN / 2) * kEpiTS;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int64_t row0 = m0 + wm * (BM / 2) + i * 32;
-            float *Cb = C + (n0 + wn * (BN / 2)) * ldc + row0 + rq;
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float4 o = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2],
-                                      acc[i][j][4 * q + 3]};
-                    *reinterpret_cast<float4 *>(T + (j * 32 + lrow) * TS + 8 * q + 4 * lgrp) = o;
-                }
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                float4 cv[8];
-#pragma unroll
-                for (int p = 0; p < 8; ++p)
-                    cv[p] = *reinterpret_cast<const float4 *>(Cb + static_cast<int64_t>((8 * h + p) * 8 + cl) * ldc);
-#pragma unroll
-                for (int p = 0; p < 8; ++p) {
-                    const float4 a = *reinterpret_cast<const float4 *>(T + ((8 * h + p) * 8 + cl) * TS + rq);
-                    float4 c4 = cv[p];
-                    c4.x -= a.x;
-                    c4.y -= a.y;
-                    c4.z -= a.z;
-                    c4.w -= a.w;
-                    *reinterpret_cast<float4 *>(Cb + static_cast<int64_t>((8 * h + p) * 8 + cl) * ldc) = c4;
-                }
-            }
-        }
+        for (int i = 0; i < 4; ++i)
+            epilogue_block_row_lds<4>(C, ldc, m0 + wm * (BM / 2) + i * 32, n0 + wn * (BN / 2), lane, T, acc[i]);
     } else {
         // one accumulator row block (16 float4 per lane) at a time with all its
         // loads in flight together; ragged edges element by element
@@ -629,12 +649,17 @@ static int schur_launch(int64_t m, int64_t n, int nslices, int kslice, const voi
     const dim3 grid(static_cast<unsigned>(tiles_m * tiles_n));
     const uint8_t *L8 = static_cast<const uint8_t *>(Lq);
     const uint8_t *U8 = static_cast<const uint8_t *>(Uq);
-    if (vec)
-        hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, true, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices,
+    const bool whole = vec && m % BM == 0 && n % BN == 0;
+    if (whole)
+        hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, 2, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices,
+                           kslice, L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
+                           static_cast<int>(tiles_m), static_cast<int>(tiles_n));
+    else if (vec)
+        hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, 1, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices,
                            kslice, L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
                            static_cast<int>(tiles_m), static_cast<int>(tiles_n));
     else
-        hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, false, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices,
+        hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, 0, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices,
                            kslice, L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
                            static_cast<int>(tiles_m), static_cast<int>(tiles_n));
     HPLAI_HIP(hipGetLastError());

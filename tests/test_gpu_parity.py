@@ -196,9 +196,12 @@ def test_quantize_bit_exact(ctx, oracle, fmt, side):
 
 
 # ------------------------------------------------------------------ Schur GEMM
-@pytest.mark.parametrize("fmt,m,n,nsl,tb", [("f16", 300, 260, 1, 128), ("f16", 1024, 896, 2, 128),
-                                            ("f16t", 1024, 768, 3, 256), ("e4m3", 768, 1024, 2, 256),
-                                            ("e4m3", 1000, 520, 1, 128),
+@pytest.mark.parametrize("fmt,m,n,nsl,tb", [("f16", 300, 260, 1, 128),
+                                            # 128 x 128 tiles, whole: C transposed through LDS
+                                            ("f16", 1024, 896, 2, 128), ("f16t", 1024, 768, 3, 256),
+                                            ("e4m3", 768, 1024, 2, 256),
+                                            # ragged, float4 C; odd ldc: element-wise C
+                                            ("e4m3", 1000, 520, 1, 128), ("f16", 1001, 384, 1, 128),
                                             # >= 512 tiles of 256 x 256: the streamed large-tile kernel,
                                             # with ragged edges in both dimensions
                                             ("f16", 5990, 5900, 2, 128), ("e4m3", 6000, 5950, 2, 256),
