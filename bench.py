@@ -33,16 +33,20 @@ def hpl_flops(n):
     return 2.0 / 3.0 * n ** 3 + 3.0 / 2.0 * n ** 2
 
 
+def _lookahead(args):
+    return 0 if args.no_lookahead else args.lookahead
+
+
 def run_form(ctx, args, fmt, outer):
     res = None
     for _ in range(args.warmup):
-        ctx.solve(args.n, args.b, fmt=fmt, outer=outer, lookahead=not args.no_lookahead, maxit=50)
+        ctx.solve(args.n, args.b, fmt=fmt, outer=outer, lookahead=_lookahead(args), maxit=50)
     import torch
     times, results = [], []
     torch.cuda.synchronize()
     w0 = time.perf_counter()
     for _ in range(args.steps):
-        r = ctx.solve(args.n, args.b, fmt=fmt, outer=outer, lookahead=not args.no_lookahead,
+        r = ctx.solve(args.n, args.b, fmt=fmt, outer=outer, lookahead=_lookahead(args),
                       maxit=50, profile=True)
         if not r["converged"]:
             raise SystemExit("%s did not reach harness < 16: %s" % (fmt, r))
@@ -213,7 +217,7 @@ def fp64_comparison(ctx, args):
     for fmt, outer in ((args.fmt, args.outer), (args.compare or None, args.compare_outer)):
         if not fmt:
             continue
-        r = ctx.solve(n, args.b, fmt=fmt, outer=outer, lookahead=not args.no_lookahead, keep_factors=True)
+        r = ctx.solve(n, args.b, fmt=fmt, outer=outer, lookahead=_lookahead(args), keep_factors=True)
         x = torch.from_numpy(ctx.solution_x(n))
         ctx.release()
         forms[fmt] = (r, x)
@@ -246,6 +250,8 @@ def main():
     ap.add_argument("--compare", default="f16", help="second form run beside the primary ('' = none)")
     ap.add_argument("--compare-outer", type=int, default=16)
     ap.add_argument("--no-lookahead", action="store_true")
+    ap.add_argument("--lookahead", type=int, default=1,
+                    help="1: next panel on a second stream; 2: also the next block row")
     ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--cpu-b", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
@@ -310,7 +316,7 @@ def main():
         "config": {"workload": "hpl-ai solve n=%d b=%d, Schur %s (outer %d), FP32 panel, FP64 IR"
                                % (args.n, args.b, args.fmt, args.outer),
                    "n": args.n, "b": args.b, "fmt": args.fmt, "outer": args.outer,
-                   "lookahead": not args.no_lookahead},
+                   "lookahead": _lookahead(args)},
         "time_to_harness_s": primary["time_s"],
         "ir_steps": primary["ir_steps"],
         "harness": primary["harness"],

@@ -24,7 +24,9 @@
 // block once instead of once per inner block, which cuts their HBM traffic.
 // With lookahead, step 3 first updates the next outer panel's columns, then
 // the panel factorization of step k+1 runs on a second stream while the rest
-// of step 3 runs on the main stream.
+// of step 3 runs on the main stream; with the row look-ahead (lookahead 2)
+// step 3 next updates the rows of block row k+1, and step 2 of k+1 follows
+// the panel on the second stream while the main stream updates the rows below.
 #include <chrono>
 #include <cfloat>
 #include <cmath>
@@ -323,7 +325,7 @@ struct LU {
     // left-looking update, lazy initialisation, TRSM and quantization) go to
     // the bulk stream, which waits for the chain only where it needs its
     // results.  s waits for the bulk stream at the end.
-    int panel(int64_t c0, int sw, int buf, hipStream_t s) const
+    int panel(int64_t c0, int sw, int buf, hipStream_t s, bool join = true) const
     {
         int rc = HPLAI_OK;
         const int64_t Bw = static_cast<int64_t>(sw) * b;
@@ -412,7 +414,7 @@ struct LU {
         }
         if (mb > 0) {
             HPLAI_HIP(hipEventRecord(ev_join, q));
-            HPLAI_HIP(hipStreamWaitEvent(s, ev_join, 0));
+            if (join) HPLAI_HIP(hipStreamWaitEvent(s, ev_join, 0));
         }
         return rc;
     }
@@ -444,13 +446,19 @@ struct LU {
         return rc;
     }
 
-    int run(bool lookahead) const
+    // lookahead 0: update, then the next panel, all on the main stream.
+    // lookahead 1: the next outer panel's columns first, then the panel on the
+    // side stream while the main stream updates the rest.
+    // lookahead 2: as 1, and the rest is updated in two parts: the rows of
+    // the next block row first, so that the side stream computes that block
+    // row (after the panel chain) while the main stream updates the rows below.
+    int run(int lookahead) const
     {
-        const int64_t Bfull = static_cast<int64_t>(outer) * b;
         int rc = HPLAI_OK;
         int64_t c0 = 0;
         int step = 0;
         int sw = static_cast<int>(std::min<int64_t>(outer, (n - c0) / b));
+        bool row_done = false;   // block row of the current step computed ahead
         if (epochs) epochs->start(main);
         rc = panel(c0, sw, 0, main);
         while (rc == HPLAI_OK && c0 < n) {
@@ -460,7 +468,8 @@ struct LU {
                 epochs->flush(false);
             }
             const int64_t Bw = static_cast<int64_t>(sw) * b;
-            rc = block_row(c0, sw, buf, main);
+            if (!row_done) rc = block_row(c0, sw, buf, main);
+            row_done = false;
             if (rc) break;
             const int64_t r0 = c0 + Bw, rest = n - r0;
             const int64_t c1 = r0;
@@ -469,15 +478,32 @@ struct LU {
                 if (lookahead) {
                     // next panel's columns first, then hand them to the side stream
                     const int64_t B1 = static_cast<int64_t>(sw1) * b;
+                    const bool ahead = lookahead > 1 && rest > B1;
                     rc = gemm(buf, 0, sw, r0, r0, rest, B1, main);
                     if (rc) break;
                     HPLAI_HIP(hipEventRecord(ev[0], main));
                     HPLAI_HIP(hipStreamWaitEvent(side, ev[0], 0));
-                    rc = panel(c1, sw1, buf ^ 1, side);
+                    // with the row look-ahead the main stream joins the panel's
+                    // bulk stream itself (the block row does not need its rows)
+                    rc = panel(c1, sw1, buf ^ 1, side, !ahead);
                     if (rc) break;
-                    HPLAI_HIP(hipEventRecord(ev[1], side));
-                    rc = gemm(buf, 0, sw, r0, r0 + B1, rest, rest - B1, main);
-                    if (rc) break;
+                    if (ahead) {
+                        rc = gemm(buf, 0, sw, r0, r0 + B1, B1, rest - B1, main);
+                        if (rc) break;
+                        HPLAI_HIP(hipEventRecord(ev[2], main));
+                        HPLAI_HIP(hipStreamWaitEvent(side, ev[2], 0));
+                        rc = block_row(c1, sw1, buf ^ 1, side);
+                        if (rc) break;
+                        HPLAI_HIP(hipEventRecord(ev[1], side));
+                        rc = gemm(buf, 0, sw, r0 + B1, r0 + B1, rest - B1, rest - B1, main);
+                        if (rc) break;
+                        HPLAI_HIP(hipStreamWaitEvent(main, ev[6], 0));   // the panel's bulk stream
+                        row_done = true;
+                    } else {
+                        HPLAI_HIP(hipEventRecord(ev[1], side));
+                        rc = gemm(buf, 0, sw, r0, r0 + B1, rest, rest - B1, main);
+                        if (rc) break;
+                    }
                     HPLAI_HIP(hipStreamWaitEvent(main, ev[1], 0));
                 } else {
                     rc = gemm(buf, 0, sw, r0, r0, rest, rest, main);
@@ -488,7 +514,6 @@ struct LU {
             c0 = c1;
             sw = sw1;
             ++step;
-            (void)Bfull;
         }
         return rc;
     }
@@ -597,7 +622,7 @@ int hplai_factor_f32(hplai_ctx ctx, int64_t n, int b, int fmt, int outer, int lo
     init_lu(lu, ctx, n, lda, b, fmt, outer > 0 ? outer : 1, false, 0, &B, &timer);
     B.A = A;   // the caller's matrix: not freed below
     rc = alloc_lu_buffers(lu, B, nullptr);
-    if (rc == HPLAI_OK) rc = lu.run(lookahead != 0);
+    if (rc == HPLAI_OK) rc = lu.run(lookahead);
     if (hipDeviceSynchronize() != hipSuccess && rc == HPLAI_OK) rc = hip_fail(hipGetLastError(), "hplai_factor_f32");
     if (rc == HPLAI_OK) rc = check_pivots(lu, B, "hplai_factor_f32");
     B.A = nullptr;
@@ -742,7 +767,7 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     res->time_gen = t1 - t0;
 
     // ---- LU ----
-    if (rc == HPLAI_OK) rc = lu.run(p->lookahead != 0);
+    if (rc == HPLAI_OK) rc = lu.run(p->lookahead);
     HPLAI_HIP(hipDeviceSynchronize());
     epochs.flush(true);
     double t2 = now_s();

@@ -367,19 +367,40 @@ constexpr int kTriStrip = 64;              // rows (L) or columns (U) per workgr
 constexpr int kTriXS = kTriStrip + 1;      // LDS stride
 constexpr int kTriThreads = 512;
 
-// L(m x b) := L Ukk^-1 (update_left_panels, panel_trf.hpp:143-157) as
-// L := L Uinv: output block jb = sum_{kb <= jb} X(:, kb) Uinv(kb, jb).  A
-// workgroup owns 64 rows (two 32-row halves sharing every Uinv operand).
-template <bool VEC>
-__global__ __launch_bounds__(kTriThreads) void trsm_l_tri_kernel(int64_t m, int b, const float *__restrict__ tri,
-                                                                 float *__restrict__ L, int64_t ldl,
-                                                                 float *__restrict__ amax, int tb)
+// Strip loads into LDS: all of a thread's 16-byte loads are issued at once
+// (one memory latency per strip), and the lanes of a half-wave write 32
+// distinct banks: lane = 8 j + i (U strip) or 4 i + j (L strip) puts row
+// chunk 4 i of column j at bank (4 i + j + const) mod 32.
+// L strip (64 rows x b columns, column-major, 16-byte aligned): X[c * kTriXS + r].
+__device__ __forceinline__ void load_strip_l(const float *__restrict__ L, int64_t ldl, int64_t i0, int b,
+                                             float *X, int wave, int lane)
 {
-    __shared__ float X[kPanelMax * kTriXS];   // X[c * kTriXS + r]: column c, row i0 + r
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int l32 = lane & 31, lh = lane >> 5;
-    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kTriStrip;
-    const int nr = m - i0 < kTriStrip ? static_cast<int>(m - i0) : kTriStrip;
+    const int i = lane >> 2, j = lane & 3;    // rows 4 i .. 4 i + 3 of column j of a 4-column group
+    const int per_wave = b / 32;              // 4-column groups per wave (b / 4 groups in all)
+    float4 v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        if (u < per_wave) {
+            const int c = 4 * (wave * per_wave + u) + j;
+            v[u] = *reinterpret_cast<const float4 *>(L + static_cast<int64_t>(c) * ldl + i0 + 4 * i);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        if (u < per_wave) {
+            float *d = X + (4 * (wave * per_wave + u) + j) * kTriXS + 4 * i;
+            d[0] = v[u].x;
+            d[1] = v[u].y;
+            d[2] = v[u].z;
+            d[3] = v[u].w;
+        }
+    }
+}
+
+// the last strip of the panel (nr < 64 rows; rows past nr read as zero)
+__device__ __forceinline__ void load_strip_l_ragged(const float *__restrict__ L, int64_t ldl, int64_t i0, int nr,
+                                                    int b, float *X, int tid)
+{
     const int tot = b * kTriStrip;
     for (int base = 0; base < tot; base += kTriThreads * 16) {
         float v[16];
@@ -396,6 +417,72 @@ __global__ __launch_bounds__(kTriThreads) void trsm_l_tri_kernel(int64_t m, int 
             if (idx < tot) X[c * kTriXS + r] = r < nr ? v[u] : 0.0f;
         }
     }
+}
+
+// U strip (b rows x 64 columns, column-major, 16-byte aligned): X[r * kTriXS + c];
+// columns past nc read as zero.  Wave w loads columns 8 w .. 8 w + 7.
+__device__ __forceinline__ void load_strip_u(const float *__restrict__ U, int64_t ldu, int64_t j0, int nc, int b,
+                                             float *X, int wave, int lane)
+{
+    const int i = lane & 7, j = lane >> 3;   // rows 32 u + 4 i .. + 3 of column 8 wave + j
+    const int c = 8 * wave + j;
+    const float *col = U + (j0 + min(c, nc - 1)) * ldu;
+    const int groups = b / 32;               // 32-row groups
+    float4 v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+        if (u < groups) v[u] = *reinterpret_cast<const float4 *>(col + 32 * u + 4 * i);
+    const bool in = c < nc;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        if (u < groups) {
+            float *d = X + (32 * u + 4 * i) * kTriXS + c;
+            d[0] = in ? v[u].x : 0.0f;
+            d[kTriXS] = in ? v[u].y : 0.0f;
+            d[2 * kTriXS] = in ? v[u].z : 0.0f;
+            d[3 * kTriXS] = in ? v[u].w : 0.0f;
+        }
+    }
+}
+
+__device__ __forceinline__ void load_strip_u_scalar(const float *__restrict__ U, int64_t ldu, int64_t j0, int nc,
+                                                    int b, float *X, int tid)
+{
+    const int tot = b * kTriStrip;
+    for (int base = 0; base < tot; base += kTriThreads * 16) {
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = min(base + u * kTriThreads + tid, tot - 1);
+            const int c = idx / b, r = idx % b;
+            v[u] = U[(j0 + min(c, nc - 1)) * ldu + r];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = base + u * kTriThreads + tid;
+            const int c = idx / b, r = idx % b;
+            if (idx < tot) X[r * kTriXS + c] = c < nc ? v[u] : 0.0f;
+        }
+    }
+}
+
+// L(m x b) := L Ukk^-1 (update_left_panels, panel_trf.hpp:143-157) as
+// L := L Uinv: output block jb = sum_{kb <= jb} X(:, kb) Uinv(kb, jb).  A
+// workgroup owns 64 rows (two 32-row halves sharing every Uinv operand).
+template <bool VEC>
+__global__ __launch_bounds__(kTriThreads) void trsm_l_tri_kernel(int64_t m, int b, const float *__restrict__ tri,
+                                                                 float *__restrict__ L, int64_t ldl,
+                                                                 float *__restrict__ amax, int tb)
+{
+    __shared__ float X[kPanelMax * kTriXS];   // X[c * kTriXS + r]: column c, row i0 + r
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, lh = lane >> 5;
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kTriStrip;
+    const int nr = m - i0 < kTriStrip ? static_cast<int>(m - i0) : kTriStrip;
+    if (VEC && nr == kTriStrip)
+        load_strip_l(L, ldl, i0, b, X, wave, lane);
+    else
+        load_strip_l_ragged(L, ldl, i0, nr, b, X, tid);
     __syncthreads();
     const int nb = b / kStrip;
     const float *Up = tri + static_cast<int64_t>(nb) * nb * 1024;
@@ -409,24 +496,40 @@ __global__ __launch_bounds__(kTriThreads) void trsm_l_tri_kernel(int64_t m, int 
         for (int h = 0; h < 2; ++h)
 #pragma unroll
             for (int e = 0; e < 16; ++e) acc[h][e] = 0.0f;
-        float4 cur[4];
+        // the Uinv operand two blocks ahead (L2 latency under a full GPU)
+        float4 cur[4], nxt[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) cur[e] = bp[e];
-        for (int kb = 0; kb <= jb; ++kb) {
-            float4 nxt[4];
-            const float4 *bn = bp + static_cast<int64_t>(min(kb + 1, jb)) * 256;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) nxt[e] = bn[e];
+        for (int e = 0; e < 4; ++e) nxt[e] = bp[static_cast<int64_t>(min(1, jb)) * 256 + e];
+        for (int kb = 0; kb <= jb; ++kb) {
+            float4 nx2[4];
+            const float4 *bn = bp + static_cast<int64_t>(min(kb + 2, jb)) * 256;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) nx2[e] = bn[e];
+            float xv[2][16];
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk) {
+                const float *xr = X + (32 * kb + 2 * kk + lh) * kTriXS + l32;
+                xv[0][kk] = xr[0];
+                xv[1][kk] = xr[32];
+            }
+            // keeps the operand loads where they are: without it the compiler
+            // re-issues each global load right before its MFMA and waits on it
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
             const float bv[16] = {cur[0].x, cur[0].y, cur[0].z, cur[0].w, cur[1].x, cur[1].y, cur[1].z, cur[1].w,
                                   cur[2].x, cur[2].y, cur[2].z, cur[2].w, cur[3].x, cur[3].y, cur[3].z, cur[3].w};
 #pragma unroll
             for (int kk = 0; kk < 16; ++kk) {
-                const float *xr = X + (32 * kb + 2 * kk + lh) * kTriXS + l32;
-                acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr[0], bv[kk], acc[0], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(xr[32], bv[kk], acc[1], 0, 0, 0);
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(xv[0][kk], bv[kk], acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(xv[1][kk], bv[kk], acc[1], 0, 0, 0);
             }
 #pragma unroll
-            for (int e = 0; e < 4; ++e) cur[e] = nxt[e];
+            for (int e = 0; e < 4; ++e) {
+                cur[e] = nxt[e];
+                nxt[e] = nx2[e];
+            }
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -468,22 +571,10 @@ __global__ __launch_bounds__(kTriThreads) void trsm_u_tri_kernel(int b, int64_t 
     const int l32 = lane & 31, lh = lane >> 5;
     const int64_t j0 = static_cast<int64_t>(blockIdx.x) * kTriStrip;
     const int nc = n - j0 < kTriStrip ? static_cast<int>(n - j0) : kTriStrip;
-    const int tot = b * kTriStrip;
-    for (int base = 0; base < tot; base += kTriThreads * 16) {
-        float v[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int idx = min(base + u * kTriThreads + tid, tot - 1);
-            const int c = idx / b, r = idx % b;
-            v[u] = U[(j0 + min(c, nc - 1)) * ldu + r];
-        }
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int idx = base + u * kTriThreads + tid;
-            const int c = idx / b, r = idx % b;
-            if (idx < tot) X[r * kTriXS + c] = c < nc ? v[u] : 0.0f;
-        }
-    }
+    if (VEC)
+        load_strip_u(U, ldu, j0, nc, b, X, wave, lane);
+    else
+        load_strip_u_scalar(U, ldu, j0, nc, b, X, tid);
     __syncthreads();
     const int nb = b / kStrip;
     float vmax[2] = {0.0f, 0.0f};
@@ -496,24 +587,40 @@ __global__ __launch_bounds__(kTriThreads) void trsm_u_tri_kernel(int b, int64_t 
         for (int h = 0; h < 2; ++h)
 #pragma unroll
             for (int e = 0; e < 16; ++e) acc[h][e] = 0.0f;
-        float4 cur[4];
+        // the Linv operand two blocks ahead (L2 latency under a full GPU)
+        float4 cur[4], nxt[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) cur[e] = ap[e];
-        for (int kb = 0; kb <= rb; ++kb) {
-            float4 nxt[4];
-            const float4 *an = ap + static_cast<int64_t>(min(kb + 1, rb)) * 256;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) nxt[e] = an[e];
+        for (int e = 0; e < 4; ++e) nxt[e] = ap[static_cast<int64_t>(min(1, rb)) * 256 + e];
+        for (int kb = 0; kb <= rb; ++kb) {
+            float4 nx2[4];
+            const float4 *an = ap + static_cast<int64_t>(min(kb + 2, rb)) * 256;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) nx2[e] = an[e];
+            float xv[2][16];
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk) {
+                const float *xr = X + (32 * kb + 2 * kk + lh) * kTriXS + l32;
+                xv[0][kk] = xr[0];
+                xv[1][kk] = xr[32];
+            }
+            // keeps the operand loads where they are: without it the compiler
+            // re-issues each global load right before its MFMA and waits on it
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
             const float av[16] = {cur[0].x, cur[0].y, cur[0].z, cur[0].w, cur[1].x, cur[1].y, cur[1].z, cur[1].w,
                                   cur[2].x, cur[2].y, cur[2].z, cur[2].w, cur[3].x, cur[3].y, cur[3].z, cur[3].w};
 #pragma unroll
             for (int kk = 0; kk < 16; ++kk) {
-                const float *xr = X + (32 * kb + 2 * kk + lh) * kTriXS + l32;
-                acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], xr[0], acc[0], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], xr[32], acc[1], 0, 0, 0);
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], xv[0][kk], acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], xv[1][kk], acc[1], 0, 0, 0);
             }
 #pragma unroll
-            for (int e = 0; e < 4; ++e) cur[e] = nxt[e];
+            for (int e = 0; e < 4; ++e) {
+                cur[e] = nxt[e];
+                nxt[e] = nx2[e];
+            }
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {

@@ -241,7 +241,7 @@ class Context:
     def factor(self, n, b, A, lda, fmt="f16", outer=1, lookahead=True):
         """LU of the n x n FP32 column-major device matrix A in place."""
         code = FMTS[fmt] if isinstance(fmt, str) else fmt
-        check(lib().hplai_factor_f32(self.handle, n, b, code, outer, 1 if lookahead else 0, ptr(A), lda))
+        check(lib().hplai_factor_f32(self.handle, n, b, code, outer, int(lookahead), ptr(A), lda))
 
     def lu_solve(self, n, nrhs, bs, LU, lda, X, ldx, trans=False, stream=None):
         """X := (LU)^-1 X, or (LU)^-T X with trans; X FP64 with nrhs columns."""
@@ -255,7 +255,7 @@ class Context:
         the reference generates it) or "higham_kappa" (Fasi and Higham's signs,
         cond(A, inf) = cond exactly); cond 0 = 1000."""
         p = Params(n=n, b=b, seed=seed, fmt=FMTS[fmt] if isinstance(fmt, str) else fmt, outer=outer,
-                   lookahead=1 if lookahead else 0, lazy=1 if lazy else 0, maxit=maxit,
+                   lookahead=int(lookahead), lazy=1 if lazy else 0, maxit=maxit,
                    verbose=1 if verbose else 0, keep_factors=1 if keep_factors else 0,
                    profile=1 if profile else 0, matrix=MATRICES[matrix] if isinstance(matrix, str) else matrix,
                    cond=cond)

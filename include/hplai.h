@@ -204,7 +204,9 @@ typedef struct hplai_params {
     uint64_t seed;       /* 42 in the reference (main.cpp:174)                     */
     int      fmt;        /* HPLAI_FMT_*                                            */
     int      outer;      /* panels per trailing update (1 = reference schedule)    */
-    int      lookahead;  /* 1: factor panel k+1 on a second stream during update k */
+    int      lookahead;  /* 1: factor panel k+1 on a second stream during update k;
+                            2: also block row k+1 there (the update's rows of block
+                            row k+1 first, then the rows below)                      */
     int      lazy;       /* 1: -d lazy initialisation (lazy_init.hpp:92-227): the trailing
                             matrix starts at zero and accumulates -sum L U; each block gets
                             its generated a_ij added when it joins a panel                */

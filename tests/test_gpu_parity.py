@@ -359,8 +359,12 @@ def test_solve_ir_through_stored_inverses(ctx, oracle):
     assert np.max(np.abs(x - orc["x"])) / np.max(np.abs(orc["x"])) < 1e-10
 
 
-@pytest.mark.parametrize("fmt,outer,mode,tol", [("f16", 2, "LU_PRECOMP", 1e-4), ("e4m3", 1, "LU_FP8", 1e-4)])
-def test_solve_packed_operands_match_oracle(ctx, oracle, fmt, outer, mode, tol):
+@pytest.mark.parametrize("fmt,outer,mode,tol,lookahead", [("f16", 2, "LU_PRECOMP", 1e-4, 1),
+                                                           ("e4m3", 1, "LU_FP8", 1e-4, 1),
+                                                           # row look-ahead: block row k+1 on the side stream
+                                                           ("f16", 2, "LU_PRECOMP", 1e-4, 2),
+                                                           ("e4m3", 1, "LU_FP8", 1e-4, 2)])
+def test_solve_packed_operands_match_oracle(ctx, oracle, fmt, outer, mode, tol, lookahead):
     """b = 512: the quantized panels are stored panel-packed (256-row panels of
     64-byte K stages) and read by the streamed 256 x 256 Schur kernel
     (n = 8192 has >= 2 tiles per CU in the first updates).  The factors must
@@ -369,7 +373,7 @@ def test_solve_packed_operands_match_oracle(ctx, oracle, fmt, outer, mode, tol):
     e4m3 MFMA's 13-bit window inside each 64-product group stays far below
     the bound here).  A misplaced 16-byte chunk or stage would be O(1)."""
     n, b = 8192, 512
-    res = ctx.solve(n, b, fmt=fmt, outer=outer, keep_factors=True)
+    res = ctx.solve(n, b, fmt=fmt, outer=outer, lookahead=lookahead, keep_factors=True)
     lu = ctx.solution_factors(n)
     ctx.release()
     ref, d = oracle.lu(getattr(oracle, mode), n, b, SEED)
@@ -386,7 +390,8 @@ def test_solve_c1_lazy(ctx):
 
 @pytest.mark.parametrize("fmt,outer,lookahead,lazy", [
     ("f16", 1, False, False), ("f16", 2, True, False), ("f16t", 4, True, False),
-    ("e4m3", 1, False, False), ("e4m3", 4, True, False), ("f16", 2, True, True)])
+    ("e4m3", 1, False, False), ("e4m3", 4, True, False), ("f16", 2, True, True),
+    ("e4m3", 2, 2, False), ("f16", 2, 2, True)])
 def test_solve_b512(ctx, fmt, outer, lookahead, lazy):
     n, b = 8192, 512
     res = ctx.solve(n, b, fmt=fmt, outer=outer, lookahead=lookahead, lazy=lazy)
