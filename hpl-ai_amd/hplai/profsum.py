@@ -21,7 +21,7 @@ the run holds more than one solve.
 command.  The command must also run one plain device-to-device copy of
 BYTES bytes (bench.py --copy-check-bytes) so that the counters can be
 calibrated: the factor is what FETCH_SIZE (WRITE_SIZE) reads for that copy
-divided by BYTES.  The HBM bytes per launch of the kernel whose name contains
+(summed over its dispatches) divided by BYTES.  The HBM bytes per launch of the kernel whose name contains
 NAME_SUBSTRING are then FETCH/factor_f + WRITE/factor_w, averaged over its
 launches.
 """
@@ -66,21 +66,23 @@ def _per_dispatch(path, counter):
     return out
 
 
-def _copy_factor(per, copy_bytes):
-    # the calibration copy is the dispatch whose counter is closest to copy_bytes
-    # among the elementwise copy kernels
-    cands = [(abs(v - copy_bytes), v) for name, v in per.values()
-             if "copy" in name.lower() or "elementwise" in name.lower()]
-    if not cands:
-        raise SystemExit("no copy kernel found for calibration")
-    return min(cands)[1] / copy_bytes
+def _copy_factor(per, copy_bytes, name_hint):
+    """Counter reading of the calibration copy divided by its bytes.
+
+    The copy of copy_bytes bytes may run as several dispatches (torch splits
+    elementwise kernels whose byte offsets exceed 32 bits), so the readings of
+    all dispatches of the kernel whose name contains name_hint are summed."""
+    got = [v for name, v in per.values() if name_hint in name]
+    if not got:
+        raise SystemExit("no calibration kernel matching %r found" % name_hint)
+    return sum(got) / copy_bytes
 
 
-def traffic(fetch, write, kernel, copy_bytes):
+def traffic(fetch, write, kernel, copy_bytes, copy_kernel="MulFunctor"):
     pf = _per_dispatch(fetch, "FETCH_SIZE")
     pw = _per_dispatch(write, "WRITE_SIZE")
-    ff = _copy_factor(pf, copy_bytes)
-    fw = _copy_factor(pw, copy_bytes)
+    ff = _copy_factor(pf, copy_bytes, copy_kernel)
+    fw = _copy_factor(pw, copy_bytes, copy_kernel)
     kf = [v for name, v in pf.values() if kernel in name]
     kw = [v for name, v in pw.values() if kernel in name]
     if not kf or not kw:
@@ -123,6 +125,8 @@ def main(argv=None):
     t.add_argument("--write", required=True)
     t.add_argument("--kernel", required=True)
     t.add_argument("--copy-bytes", type=int, required=True)
+    t.add_argument("--copy-kernel", default="MulFunctor",
+                   help="name substring of the calibration copy's kernel (bench.py runs torch.mul)")
     g = sub.add_parser("launches")
     g.add_argument("csv")
     g.add_argument("--kernel", required=True)
@@ -133,7 +137,7 @@ def main(argv=None):
     elif a.cmd == "launches":
         launches(a.csv, a.kernel, a.first)
     else:
-        traffic(a.fetch, a.write, a.kernel, a.copy_bytes)
+        traffic(a.fetch, a.write, a.kernel, a.copy_bytes, a.copy_kernel)
 
 
 if __name__ == "__main__":

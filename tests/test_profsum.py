@@ -1,0 +1,43 @@
+"""CPU tests of the rocprofv3 summary helper (hpl-ai_amd/hplai/profsum.py):
+the traffic figure divides the kernel's counter readings by the calibration
+copy's reading per byte, summed over all dispatches of the copy."""
+import csv
+import os
+import sys
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "hpl-ai_amd"))
+
+from hplai import profsum  # noqa: E402
+
+FIELDS = ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"]
+
+
+def _write(path, counter, rows):
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=FIELDS)
+        w.writeheader()
+        for d, name, kib in rows:
+            w.writerow({"Dispatch_Id": d, "Kernel_Name": name, "Counter_Name": counter, "Counter_Value": kib})
+
+
+def test_traffic_sums_split_calibration_copy(tmp_path, capsys):
+    gib = 1 << 30
+    mul = "vectorized_elementwise_kernel<4, MulFunctor<float>>"
+    # a 4 GiB copy split into two dispatches; FETCH_SIZE reads half of what is read,
+    # WRITE_SIZE all of it
+    fetch = [(1, mul, gib // 1024), (2, mul, gib // 1024),
+             (3, "schur_dma_kernel<2, 2, true>", 3 * gib // 1024),
+             (4, "schur_dma_kernel<2, 2, true>", 5 * gib // 1024),
+             (5, "fill", 7)]
+    write = [(1, mul, 2 * gib // 1024), (2, mul, 2 * gib // 1024),
+             (3, "schur_dma_kernel<2, 2, true>", gib // 1024),
+             (4, "schur_dma_kernel<2, 2, true>", gib // 1024)]
+    pf, pw = tmp_path / "f.csv", tmp_path / "w.csv"
+    _write(pf, "FETCH_SIZE", fetch)
+    _write(pw, "WRITE_SIZE", write)
+    profsum.traffic(str(pf), str(pw), "schur_dma_kernel<2, 2, true>", 4 * gib)
+    out = dict(line.split(None, 1) for line in capsys.readouterr().out.strip().splitlines())
+    assert float(out["fetch_factor"]) == 0.5
+    assert float(out["write_factor"]) == 1.0
+    # mean fetch reading 4 GiB / 0.5 + mean write reading 1 GiB / 1.0
+    assert abs(float(out["hbm_bytes_per_launch"]) / (9 * gib) - 1.0) < 1e-5
