@@ -625,6 +625,120 @@ __global__ __launch_bounds__(256) void quant_u_kernel(int64_t m, int64_t k, int 
     }
 }
 
+// 16 consecutive K values of one row scaled by 2^e and stored at d (16 bytes
+// of e4m3 or 32 bytes of FP16; d 16-byte aligned).
+template <int FMT>
+__device__ __forceinline__ void store_q16(uint8_t *d, const float (&v)[16], int e)
+{
+    float x[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) x[q] = ldexp_fast(v[q], e);
+    if constexpr (FMT == HPLAI_FMT_E4M3) {
+        uint4 w;
+        w.x = f32x4_to_e4m3x4(x[0], x[1], x[2], x[3]);
+        w.y = f32x4_to_e4m3x4(x[4], x[5], x[6], x[7]);
+        w.z = f32x4_to_e4m3x4(x[8], x[9], x[10], x[11]);
+        w.w = f32x4_to_e4m3x4(x[12], x[13], x[14], x[15]);
+        *reinterpret_cast<uint4 *>(d) = w;
+    } else {
+        uint32_t h[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            h[q] = f32_to_f16_bits(x[2 * q]) | (static_cast<uint32_t>(f32_to_f16_bits(x[2 * q + 1])) << 16);
+        *reinterpret_cast<uint4 *>(d) = uint4{h[0], h[1], h[2], h[3]};
+        *reinterpret_cast<uint4 *>(d + 16) = uint4{h[4], h[5], h[6], h[7]};
+    }
+}
+
+// The same quantizers with 16 values per thread for aligned shapes (k % 16 == 0,
+// 16-byte aligned source columns and destination rows): 64-byte loads and one
+// 16-byte (e4m3) or 32-byte (FP16) store per thread.
+// L panel: a 64 x 64 block through LDS; thread p takes row p % 64 of the block
+// and K values 16 (p / 64) .. + 15, so a half-wave reads 32 distinct banks.
+template <int FMT, bool PACKED>
+__global__ __launch_bounds__(256) void quant_l16_kernel(int64_t m, int64_t k, int tb,
+                                                        const float *__restrict__ src, int64_t lds,
+                                                        void *__restrict__ dst, int64_t ldd,
+                                                        const float *__restrict__ amax,
+                                                        int32_t *__restrict__ exps)
+{
+    __shared__ float T[64][65];
+    constexpr int es = FMT == HPLAI_FMT_E4M3 ? 1 : 2;
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 64;
+    const int64_t c0 = static_cast<int64_t>(blockIdx.y) * 64;
+    const int tid = threadIdx.x;
+    // loads: 64 columns x 16 float4; thread tid takes float4 i = tid % 16 of
+    // columns tid / 16 + 16 u
+    {
+        const int i = tid & 15;
+        const int64_t gr = r0 + 4 * i;
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = (tid >> 4) + 16 * u;
+            const int64_t gc = c0 + c < k ? c0 + c : k - 1;
+            const float *col = src + gc * lds;
+            if (gr + 3 < m) {
+                v[u] = *reinterpret_cast<const float4 *>(col + gr);
+            } else {
+                v[u].x = gr < m ? col[gr] : 0.0f;
+                v[u].y = gr + 1 < m ? col[gr + 1] : 0.0f;
+                v[u].z = gr + 2 < m ? col[gr + 2] : 0.0f;
+                v[u].w = 0.0f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = (tid >> 4) + 16 * u;
+            T[c][4 * i] = v[u].x;
+            T[c][4 * i + 1] = v[u].y;
+            T[c][4 * i + 2] = v[u].z;
+            T[c][4 * i + 3] = v[u].w;
+        }
+    }
+    __syncthreads();
+    const int r = tid & 63, kc = (tid >> 6) * 16;
+    const int64_t gr = r0 + r;
+    if (gr >= m || c0 + kc >= k) return;
+    const int e = fmt_exponent<FMT>(amax, gr / tb);
+    if (exps != nullptr && (gr % tb) == 0 && c0 == 0 && kc == 0) exps[gr / tb] = e;
+    float v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = T[kc + q][r];
+    uint8_t *d = static_cast<uint8_t *>(dst) +
+                 (PACKED ? packed_off(gr, es * (c0 + kc), es * ldd) : (gr * ldd + c0 + kc) * es);
+    store_q16<FMT>(d, v, e);
+}
+
+// U panel: thread t takes K values 16 (t % (k / 16)) .. + 15 of column t / (k / 16).
+template <int FMT, bool PACKED>
+__global__ __launch_bounds__(256) void quant_u16_kernel(int64_t m, int64_t k, int tb,
+                                                        const float *__restrict__ src, int64_t lds,
+                                                        void *__restrict__ dst, int64_t ldd,
+                                                        const float *__restrict__ amax,
+                                                        int32_t *__restrict__ exps)
+{
+    constexpr int es = FMT == HPLAI_FMT_E4M3 ? 1 : 2;
+    const int64_t kq = k / 16;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t j = t / kq, r = (t % kq) * 16;
+    if (j >= m) return;
+    const int e = fmt_exponent<FMT>(amax, j / tb);
+    if (exps != nullptr && r == 0 && (j % tb) == 0) exps[j / tb] = e;
+    const float4 *s4 = reinterpret_cast<const float4 *>(src + j * lds + r);
+    float v[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 x = s4[q];
+        v[4 * q] = x.x;
+        v[4 * q + 1] = x.y;
+        v[4 * q + 2] = x.z;
+        v[4 * q + 3] = x.w;
+    }
+    uint8_t *d = static_cast<uint8_t *>(dst) + (PACKED ? packed_off(j, es * r, es * ldd) : (j * ldd + r) * es);
+    store_q16<FMT>(d, v, e);
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -713,10 +827,21 @@ template <int FMT, bool PACKED>
 static int quant_dispatch(int side, int64_t m, int64_t k, int tb, const float *src, int64_t lds,
                           void *dst, int64_t ldd, const float *amax, int32_t *exps, hipStream_t s)
 {
-    if (side == 0) {
+    constexpr int es = FMT == HPLAI_FMT_E4M3 ? 1 : 2;
+    const bool v16 = k % 16 == 0 && lds % 4 == 0 && reinterpret_cast<uintptr_t>(src) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(dst) % 16 == 0 && (ldd * es) % 16 == 0;
+    if (side == 0 && v16) {
+        dim3 grid(static_cast<unsigned>(ceil_div(m, 64)), static_cast<unsigned>(ceil_div(k, 64)));
+        hipLaunchKernelGGL((quant_l16_kernel<FMT, PACKED>), grid, dim3(256), 0, s, m, k, tb, src, lds, dst, ldd,
+                           amax, exps);
+    } else if (side == 0) {
         dim3 grid(static_cast<unsigned>(ceil_div(m, 64)), static_cast<unsigned>(ceil_div(k, 64)));
         hipLaunchKernelGGL((quant_l_kernel<FMT, PACKED>), grid, dim3(256), 0, s, m, k, tb, src, lds, dst, ldd,
                            amax, exps);
+    } else if (v16) {
+        const int64_t thr = m * (k / 16);
+        hipLaunchKernelGGL((quant_u16_kernel<FMT, PACKED>), dim3(static_cast<unsigned>(ceil_div(thr, 256))),
+                           dim3(256), 0, s, m, k, tb, src, lds, dst, ldd, amax, exps);
     } else {
         const int64_t thr = m * ceil_div(k, 4);
         hipLaunchKernelGGL((quant_u_kernel<FMT, PACKED>), dim3(static_cast<unsigned>(ceil_div(thr, 256))),

@@ -152,33 +152,39 @@ def _panel(rng, m, k, scale_rows=None):
 
 @pytest.mark.parametrize("fmt", ["f16", "f16t", "e4m3"])
 @pytest.mark.parametrize("side", [0, 1])
-def test_quantize_bit_exact(ctx, oracle, fmt, side):
+@pytest.mark.parametrize("cut", [37, 38])
+def test_quantize_bit_exact(ctx, oracle, fmt, side, cut):
+    """cut 37: odd row count, element-wise L-panel path; cut 38: 16 K values
+    per thread (float4 loads, 16/32-byte stores) with a ragged last row group."""
     import hplai
     import torch
     rng = np.random.default_rng(7)
     tb, ntile = 128, 5
-    m, k = tb * ntile - 37, 128       # m rows (side 0) or columns (side 1)
+    m, k = tb * ntile - cut, 128       # m rows (side 0) or columns (side 1)
     # tiles with very different magnitudes exercise the per-tile exponents
     mags = 10.0 ** rng.integers(-6, 3, size=ntile)
     rowscale = np.repeat(mags, tb)[:m]
     if side == 0:
         X = _panel(rng, m, k, rowscale)            # X[c, r]: m x k column-major
-        src = X
+        lds = -(-m // 4) * 4                       # 16-byte aligned columns
+        src = np.zeros((k, lds), dtype=np.float32)
+        src[:, :m] = X
     else:
         X = _panel(rng, k, m)                      # X[j, r]: k x m column-major (k rows)
         X *= rowscale[:, None].astype(np.float32)
-        src = X
+        src, lds = X, k
+    if cut == 37:
+        src, lds = X, (m if side == 0 else k)
     es = 1 if fmt == "e4m3" else 2
     ldd = k
     dst = _zeros(m * ldd * es, torch.uint8)
     exps = _zeros(ntile, torch.int32)
     code = hplai.FMTS[fmt]
-    ctx.quantize(code, side, m, k, tb, _t(src), m if side == 0 else k, dst, ldd,
-                 exps if fmt != "f16" else None)
+    ctx.quantize(code, side, m, k, tb, _t(src), lds, dst, ldd, exps if fmt != "f16" else None)
     _sync()
     got = dst.cpu().numpy()
     # expected: row i of the panel (K-contiguous) scaled by 2^e of its tile
-    rows = src.T if side == 0 else src            # rows[i, kk]
+    rows = X.T if side == 0 else X                # rows[i, kk]
     e = np.zeros(ntile, dtype=np.int64)
     if fmt != "f16":
         fc = 2 if fmt == "e4m3" else 1
