@@ -208,15 +208,39 @@ __global__ __launch_bounds__(kGetrfThreads) void getrf_mfma_kernel(int n, float 
     for (int k = 0; k < n; k += W) {
         const int h = n - k, wr = n - k - W;
         float *Ak = A + static_cast<int64_t>(k) * lda + k;
-        // 1. stage the panel (h x 32) and the U strip (32 x wr)
-        for (int e = tid; e < h * (W / 4); e += NT) {
-            const int c = e / (h / 4), r4 = (e % (h / 4)) * 4;
-            const float4 v = *reinterpret_cast<const float4 *>(Ak + static_cast<int64_t>(c) * lda + r4);
-            *reinterpret_cast<float4 *>(&Lc[c * kLcS + r4]) = v;
-        }
-        for (int e = tid; e < wr * W; e += NT) {
-            const int c = e / W, r = e % W;
-            Us[c * 33 + r] = Ak[static_cast<int64_t>(W + c) * lda + r];
+        // 1. stage the panel (h x 32) and the U strip (32 x wr): every load of
+        //    the step is issued before the first LDS write (one L2 latency per
+        //    step instead of one per loop trip).  Panel: thread takes row chunk
+        //    tid % 128 of columns tid / 128 + 4 u; U strip: row chunk tid % 8 of
+        //    columns tid / 8 + 64 u.
+        {
+            const int pf = tid & 127, pg = tid >> 7;
+            const int ui = tid & 7, uc = tid >> 3;
+            const bool pin = 4 * pf < h;
+            const float *pcol = Ak + static_cast<int64_t>(pg) * lda + 4 * pf;
+            const float *ucol = Ak + static_cast<int64_t>(W + uc) * lda + 4 * ui;
+            float4 vp[8], vu[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                vp[u] = pin ? *reinterpret_cast<const float4 *>(pcol + static_cast<int64_t>(4 * u) * lda)
+                            : float4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                vu[u] = uc + 64 * u < wr ? *reinterpret_cast<const float4 *>(ucol + static_cast<int64_t>(64 * u) * lda)
+                                         : float4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (pin) *reinterpret_cast<float4 *>(&Lc[(pg + 4 * u) * kLcS + 4 * pf]) = vp[u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (uc + 64 * u < wr) {
+                    float *d = &Us[(uc + 64 * u) * 33 + 4 * ui];
+                    d[0] = vu[u].x;
+                    d[1] = vu[u].y;
+                    d[2] = vu[u].z;
+                    d[3] = vu[u].w;
+                }
+            }
         }
         __syncthreads();
         // 2. LU of the 32 x 32 diagonal block by wave 0
