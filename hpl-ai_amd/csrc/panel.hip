@@ -366,21 +366,30 @@ __global__ __launch_bounds__(kGetrfThreads) void getrf_mfma_kernel(int n, float 
             }
         }
         __syncthreads();
-        // 5. trailing update A22 -= L21 U12 (hr x wr, both multiples of 32)
-        for (int t0 = wave * 4; t0 < TR * TC; t0 += 4 * (NT / 64)) {
-            float4 cv[4][4];
+        // 5. trailing update A22 -= L21 U12 (hr x wr, both multiples of 32):
+        //    two tiles per wave at a time; the C of the wave's next two tiles
+        //    loads while these two are computed
+        const int ntile = TR * TC;
+        constexpr int TPW = 2, STEP = TPW * (NT / 64);
+        auto load_c = [&](int t0, float4 (&v)[TPW][4]) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int t = t0 + u < TR * TC ? t0 + u : TR * TC - 1;
+            for (int u = 0; u < TPW; ++u) {
+                const int t = t0 + u < ntile ? t0 + u : ntile - 1;
                 const int ti = t % TR, tj = t / TR;
                 const float *ccol = Ak + static_cast<int64_t>(W + tj * 32 + l32) * lda + W + ti * 32 + 4 * lh;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) cv[u][q] = *reinterpret_cast<const float4 *>(ccol + 8 * q);
+                for (int q = 0; q < 4; ++q) v[u][q] = *reinterpret_cast<const float4 *>(ccol + 8 * q);
             }
+        };
+        float4 cv[TPW][4];
+        if (wave * TPW < ntile) load_c(wave * TPW, cv);
+        for (int t0 = wave * TPW; t0 < ntile; t0 += STEP) {
+            float4 cn[TPW][4];
+            if (t0 + STEP < ntile) load_c(t0 + STEP, cn);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < TPW; ++u) {
                 const int t = t0 + u;
-                if (t >= TR * TC) break;
+                if (t >= ntile) break;
                 const int ti = t % TR, tj = t / TR;
                 floatx16g_t acc;
 #pragma unroll
@@ -402,6 +411,10 @@ __global__ __launch_bounds__(kGetrfThreads) void getrf_mfma_kernel(int n, float 
                     *reinterpret_cast<float4 *>(ccol + 8 * q) = c4;
                 }
             }
+#pragma unroll
+            for (int u = 0; u < TPW; ++u)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) cv[u][q] = cn[u][q];
         }
         __syncthreads();
     }
