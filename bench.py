@@ -37,17 +37,22 @@ def _lookahead(args):
     return 0 if args.no_lookahead else args.lookahead
 
 
+def _storage(args):
+    """lazy initialisation / FP16 trailing storage options of the solve"""
+    return dict(lazy=bool(args.lazy or args.trail16), trail16=bool(args.trail16))
+
+
 def run_form(ctx, args, fmt, outer):
     res = None
     for _ in range(args.warmup):
-        ctx.solve(args.n, args.b, fmt=fmt, outer=outer, lookahead=_lookahead(args), maxit=50)
+        ctx.solve(args.n, args.b, fmt=fmt, outer=outer, lookahead=_lookahead(args), maxit=50, **_storage(args))
     import torch
     times, results = [], []
     torch.cuda.synchronize()
     w0 = time.perf_counter()
     for _ in range(args.steps):
         r = ctx.solve(args.n, args.b, fmt=fmt, outer=outer, lookahead=_lookahead(args),
-                      maxit=50, profile=True)
+                      maxit=50, profile=True, **_storage(args))
         if not r["converged"]:
             raise SystemExit("%s did not reach harness < 16: %s" % (fmt, r))
         times.append(r["time_total"])
@@ -250,6 +255,9 @@ def main():
     ap.add_argument("--compare", default="f16", help="second form run beside the primary ('' = none)")
     ap.add_argument("--compare-outer", type=int, default=16)
     ap.add_argument("--no-lookahead", action="store_true")
+    ap.add_argument("--lazy", action="store_true", help="lazy initialisation (-d)")
+    ap.add_argument("--trail16", action="store_true",
+                    help="FP16 trailing storage of the partial sums (implies --lazy)")
     ap.add_argument("--lookahead", type=int, default=1,
                     help="1: next panel on a second stream; 2: also the next block row")
     ap.add_argument("--cpu-n", type=int, default=16384)
@@ -313,10 +321,12 @@ def main():
         "vs_baseline": None,
         "dtype": "e4m3" if args.fmt == "e4m3" else "f16",
         "data": "synthetic (seeded HPL-AI LCG matrix, generated on the GPU)",
-        "config": {"workload": "hpl-ai solve n=%d b=%d, Schur %s (outer %d), FP32 panel, FP64 IR"
-                               % (args.n, args.b, args.fmt, args.outer),
+        "config": {"workload": "hpl-ai solve n=%d b=%d, Schur %s (outer %d), FP32 panel, FP64 IR%s"
+                               % (args.n, args.b, args.fmt, args.outer,
+                                  ", lazy init, FP16 trailing storage" if args.trail16
+                                  else (", lazy init" if args.lazy else "")),
                    "n": args.n, "b": args.b, "fmt": args.fmt, "outer": args.outer,
-                   "lookahead": _lookahead(args)},
+                   "lookahead": _lookahead(args), **_storage(args)},
         "time_to_harness_s": primary["time_s"],
         "ir_steps": primary["ir_steps"],
         "harness": primary["harness"],

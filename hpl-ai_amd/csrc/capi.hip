@@ -184,6 +184,18 @@ int hplai_schur_update(hplai_ctx ctx, int fmt, int64_t m, int64_t n, int nslices
                         ctx->num_cu, as_stream(stream));
 }
 
+int hplai_schur_update_f16c(hplai_ctx ctx, int fmt, int64_t m, int64_t n, int nslices, int kslice,
+                            const void *Lq, int64_t ldl, const int32_t *eL, int64_t ldel, const void *Uq,
+                            int64_t ldu, const int32_t *eU, int64_t ldeu, int tb, uint16_t *C, int64_t ldc,
+                            float cscale, void *stream)
+{
+    CTX_CHECK(ctx);
+    HPLAI_CHECK(m >= 0 && n >= 0 && nslices >= 0 && Lq != nullptr && Uq != nullptr && C != nullptr,
+                "schur_update_f16c: bad shape/pointers");
+    return launch_schur_c16(fmt, m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu, tb, C, ldc, cscale,
+                            ctx->num_cu, as_stream(stream));
+}
+
 int hplai_otf_gemv_f64(hplai_ctx ctx, int64_t n, uint64_t seed, const double *d, double alpha,
                        const double *x, double *y, void *stream)
 {

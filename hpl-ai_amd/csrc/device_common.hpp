@@ -72,6 +72,11 @@ __device__ __forceinline__ double lcg_value(uint64_t s)
 // ---------------------------------------------------------------------------
 // Low-precision helpers
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ float f16_bits_to_f32(uint32_t h)
+{
+    return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(h)));
+}
+
 __device__ __forceinline__ uint16_t f32_to_f16_bits(float x)
 {
     _Float16 h = static_cast<_Float16>(x);   // v_cvt_f16_f32, round to nearest even

@@ -41,9 +41,12 @@ inline hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ---- kernel launchers (no argument validation; callers validate) ----
-// accumulate != 0: A += generated block (lazy initialisation)
+// accumulate != 0: A += generated block (lazy initialisation); P (FP16,
+// leading dimension ldp, may be NULL): A += pscale * P as well (the FP16
+// trailing storage's partial sums of the block)
 int launch_gen_matrix(int64_t n, uint64_t seed, int64_t i0, int64_t j0, int64_t m, int64_t k,
-                      const double *d, float *A, int64_t lda, int accumulate, hipStream_t s);
+                      const double *d, float *A, int64_t lda, int accumulate, hipStream_t s,
+                      const uint16_t *P = nullptr, int64_t ldp = 0, float pscale = 0.0f);
 int launch_gen_diag(int64_t n, uint64_t seed, int64_t i0, int64_t cnt, double *d, hipStream_t s);
 // Higham's HPL-AI matrix (higham.hip)
 int higham_coeffs(int64_t n, double cond, double rho, double *alpha, double *beta);
@@ -80,6 +83,12 @@ int launch_schur(int fmt, int64_t m, int64_t n, int nslices, int kslice, const v
                  int64_t ldl, const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
                  const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, int num_cu,
                  hipStream_t s, bool packed = false);
+// The same update of the FP16 trailing storage: C := fp16(C - cscale * L U),
+// C holding cscale times the partial sums (m, n multiples of 128).
+int launch_schur_c16(int fmt, int64_t m, int64_t n, int nslices, int kslice, const void *Lq,
+                     int64_t ldl, const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
+                     const int32_t *eU, int64_t ldeu, int tb, uint16_t *C, int64_t ldc, float cscale,
+                     int num_cu, hipStream_t s, bool packed = false);
 // true when launch_schur runs the streamed 256 x 256 kernel for this shape
 bool schur_large_tiles(int fmt, int64_t m, int64_t n, int tb, int num_cu);
 

@@ -16,7 +16,8 @@ constexpr int kGenCols = 64;    // columns per thread: amortises the jump-ahead
 // fill_one_panel_with_rand (hpl_rand.hpp:59-106) for a block at (i0, j0).
 __global__ __launch_bounds__(kGenRows) void gen_matrix_kernel(
     int64_t n, uint64_t x1, int64_t i0, int64_t j0, int64_t m, int64_t k, int64_t row_blocks,
-    LcgCoeff jn, const double *__restrict__ d, float *__restrict__ A, int64_t lda, int accumulate)
+    LcgCoeff jn, const double *__restrict__ d, float *__restrict__ A, int64_t lda, int accumulate,
+    const uint16_t *__restrict__ P, int64_t ldp, float pscale)
 {
     const int64_t rb = blockIdx.x % row_blocks;
     const int64_t cg = blockIdx.x / row_blocks;
@@ -28,11 +29,17 @@ __global__ __launch_bounds__(kGenRows) void gen_matrix_kernel(
     uint64_t s = lcg_jump(x1, static_cast<uint64_t>(gi) +
                                   static_cast<uint64_t>(n) * static_cast<uint64_t>(j0 + jl0));
     float *col = A + jl0 * lda + il;
+    const uint16_t *pcol = P != nullptr ? P + jl0 * ldp + il : nullptr;
     for (int64_t jl = jl0; jl < jl1; ++jl, col += lda) {
         float v = static_cast<float>(lcg_value(s));
         if (d != nullptr && gi == j0 + jl) v = static_cast<float>(d[gi]);
-        // lazy initialisation merges the partial sum: to = from + a_ij
-        // (LazyInitializer::update, lazy_init.hpp:147-227, with scale 1)
+        // lazy initialisation merges the partial sum: to = scale * from + a_ij
+        // (LazyInitializer::update, lazy_init.hpp:147-227); the FP16 trailing
+        // storage holds the partial sums of earlier outer steps times 1 / scale
+        if (pcol != nullptr) {
+            v = fmaf(pscale, f16_bits_to_f32(*pcol), v);
+            pcol += ldp;
+        }
         *col = accumulate ? *col + v : v;
         s = jn.a * s + jn.c;
     }
@@ -89,7 +96,8 @@ __global__ void gen_rhs_kernel(int64_t n, uint64_t x1, int64_t i0, int64_t cnt,
 }
 
 int launch_gen_matrix(int64_t n, uint64_t seed, int64_t i0, int64_t j0, int64_t m, int64_t k,
-                      const double *d, float *A, int64_t lda, int accumulate, hipStream_t s)
+                      const double *d, float *A, int64_t lda, int accumulate, hipStream_t s,
+                      const uint16_t *P, int64_t ldp, float pscale)
 {
     if (m <= 0 || k <= 0) return HPLAI_OK;
     const int64_t row_blocks = ceil_div(m, kGenRows);
@@ -103,7 +111,8 @@ int launch_gen_matrix(int64_t n, uint64_t seed, int64_t i0, int64_t j0, int64_t 
         int64_t kk = (k - jj) < g * kGenCols ? (k - jj) : g * kGenCols;
         hipLaunchKernelGGL(gen_matrix_kernel, dim3(static_cast<unsigned>(row_blocks * g)),
                            dim3(kGenRows), 0, s, n, lcg_init(seed), i0, j0 + jj, m, kk, row_blocks,
-                           jn, d, A + jj * lda, lda, accumulate);
+                           jn, d, A + jj * lda, lda, accumulate, P != nullptr ? P + jj * ldp : nullptr, ldp,
+                           pscale);
         HPLAI_HIP(hipGetLastError());
     }
     return HPLAI_OK;

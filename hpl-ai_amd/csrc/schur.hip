@@ -150,16 +150,58 @@ __device__ __forceinline__ void epilogue_block_row_lds(float *__restrict__ C, in
     }
 }
 
+// The same with the trailing matrix in FP16 (16-bit trailing storage): C
+// holds cscale times the partial sums, C := fp16(C - cscale * acc), each lane
+// reading and writing 4 rows (8 bytes) of a column.
+template <int JB>
+__device__ __forceinline__ void epilogue_block_row_lds_h(uint16_t *__restrict__ C, int64_t ldc, int64_t row0,
+                                                         int64_t col0, int lane, float *T,
+                                                         const floatx16_t (&acc)[JB], float cscale)
+{
+    const int lrow = lane & 31, lgrp = lane >> 5;
+    const int cl = lane >> 3, rq = (lane & 7) * 4;
+#pragma unroll
+    for (int j = 0; j < JB; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 o = {acc[j][4 * q], acc[j][4 * q + 1], acc[j][4 * q + 2], acc[j][4 * q + 3]};
+            *reinterpret_cast<float4 *>(T + (j * 32 + lrow) * kEpiTS + 8 * q + 4 * lgrp) = o;
+        }
+    uint16_t *Cb = C + col0 * ldc + row0 + rq;
+    constexpr int PASSES = JB * 32 / 8;
+#pragma unroll
+    for (int h = 0; h < PASSES / 8; ++h) {
+        uint2 cv[8];
+#pragma unroll
+        for (int p = 0; p < 8; ++p)
+            cv[p] = *reinterpret_cast<const uint2 *>(Cb + static_cast<int64_t>((8 * h + p) * 8 + cl) * ldc);
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            const float4 a = *reinterpret_cast<const float4 *>(T + ((8 * h + p) * 8 + cl) * kEpiTS + rq);
+            const float c0 = f16_bits_to_f32(cv[p].x & 0xffffu) - cscale * a.x;
+            const float c1 = f16_bits_to_f32(cv[p].x >> 16) - cscale * a.y;
+            const float c2 = f16_bits_to_f32(cv[p].y & 0xffffu) - cscale * a.z;
+            const float c3 = f16_bits_to_f32(cv[p].y >> 16) - cscale * a.w;
+            uint2 o;
+            o.x = f32_to_f16_bits(c0) | (static_cast<uint32_t>(f32_to_f16_bits(c1)) << 16);
+            o.y = f32_to_f16_bits(c2) | (static_cast<uint32_t>(f32_to_f16_bits(c3)) << 16);
+            *reinterpret_cast<uint2 *>(Cb + static_cast<int64_t>((8 * h + p) * 8 + cl) * ldc) = o;
+        }
+    }
+}
+
 // EPI: 0 = element-wise C access (unaligned C), 1 = float4 per accumulator
 // row block with ragged edges, 2 = whole tiles only (M, N multiples of the
-// tile), C transposed through LDS (epilogue_block_row_lds).
+// tile), C transposed through LDS (epilogue_block_row_lds), 3 = as 2 with an
+// FP16 C scaled by cscale (epilogue_block_row_lds_h).
 template <int FMT, int BM, int BN, int EPI, bool PACKED>
 __global__ __launch_bounds__(kThreads) void schur_kernel(
     int64_t M, int64_t N, int nslices, int kslice, const uint8_t *__restrict__ Lq, int64_t ldl_b,
     const int32_t *__restrict__ eL, int64_t ldel, const uint8_t *__restrict__ Uq, int64_t ldu_b,
-    const int32_t *__restrict__ eU, int64_t ldeu, int tb, float *__restrict__ C, int64_t ldc,
-    int tiles_m, int tiles_n)
+    const int32_t *__restrict__ eU, int64_t ldeu, int tb, void *__restrict__ Cv, int64_t ldc,
+    int tiles_m, int tiles_n, float cscale)
 {
+    float *__restrict__ C = static_cast<float *>(Cv);
     using F = SchurFmt<FMT>;
     constexpr int IB = BM / 64, JB = BN / 64;          // 32 x 32 blocks per wave
     constexpr int QA = BM * 8 / kThreads, QB = BN * 8 / kThreads;   // 16-byte chunks per thread
@@ -341,13 +383,19 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
 #pragma unroll
                 for (int r = 0; r < 16; ++r) acc[i][j][r] = ldexp_fast(acc[i][j][r], -e_cur);
     }
-    if constexpr (EPI == 2) {
+    if constexpr (EPI >= 2) {
         static_assert(4 * (BN / 2) * kEpiTS * 4 <= 2 * BM * kLdsRow, "epilogue scratch fits the A stages");
         __syncthreads();   // every wave is done with the operand stages
         float *T = reinterpret_cast<float *>(&As[0][0]) + wave * (BN / 2) * kEpiTS;
 #pragma unroll
-        for (int i = 0; i < IB; ++i)
-            epilogue_block_row_lds<JB>(C, ldc, m0 + wm * (BM / 2) + i * 32, n0 + wn * (BN / 2), lane, T, acc[i]);
+        for (int i = 0; i < IB; ++i) {
+            if constexpr (EPI == 3)
+                epilogue_block_row_lds_h<JB>(static_cast<uint16_t *>(Cv), ldc, m0 + wm * (BM / 2) + i * 32,
+                                             n0 + wn * (BN / 2), lane, T, acc[i], cscale);
+            else
+                epilogue_block_row_lds<JB>(C, ldc, m0 + wm * (BM / 2) + i * 32, n0 + wn * (BN / 2), lane, T,
+                                           acc[i]);
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < IB; ++i)
@@ -381,9 +429,10 @@ template <int FMT, int EPI, bool PACKED>
 __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
     int64_t M, int64_t N, int nslices, int kslice, const uint8_t *__restrict__ Lq, int64_t ldl_b,
     const int32_t *__restrict__ eL, int64_t ldel, const uint8_t *__restrict__ Uq, int64_t ldu_b,
-    const int32_t *__restrict__ eU, int64_t ldeu, int tb, float *__restrict__ C, int64_t ldc,
-    int tiles_m, int tiles_n)
+    const int32_t *__restrict__ eU, int64_t ldeu, int tb, void *__restrict__ Cv, int64_t ldc,
+    int tiles_m, int tiles_n, float cscale)
 {
+    float *__restrict__ C = static_cast<float *>(Cv);
     static_assert(FMT != HPLAI_FMT_F16_TILE, "per-tile FP16 rescales accumulators; not supported here");
     static_assert(kDmaLoadsPerStage == 8, "four Lq and four Uq loads per thread per stage");
     using F = SchurFmt<FMT>;
@@ -584,12 +633,18 @@ __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 
     // ---- epilogue: C -= acc ----
-    if constexpr (EPI == 2) {
+    if constexpr (EPI >= 2) {
         __syncthreads();   // every wave is done with the operand stages
         float *T = reinterpret_cast<float *>(S) + wave * (BN / 2) * kEpiTS;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            epilogue_block_row_lds<4>(C, ldc, m0 + wm * (BM / 2) + i * 32, n0 + wn * (BN / 2), lane, T, acc[i]);
+        for (int i = 0; i < 4; ++i) {
+            if constexpr (EPI == 3)
+                epilogue_block_row_lds_h<4>(static_cast<uint16_t *>(Cv), ldc, m0 + wm * (BM / 2) + i * 32,
+                                            n0 + wn * (BN / 2), lane, T, acc[i], cscale);
+            else
+                epilogue_block_row_lds<4>(C, ldc, m0 + wm * (BM / 2) + i * 32, n0 + wn * (BN / 2), lane, T,
+                                          acc[i]);
+        }
     } else {
         // one accumulator row block (16 float4 per lane) at a time with all its
         // loads in flight together; ragged edges element by element
@@ -600,10 +655,12 @@ __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
     }
 }
 
+// c16: C is the FP16 trailing storage (whole tiles, 8-byte aligned), C := fp16(C - cscale * L U)
 template <int FMT, bool PACKED>
 static int schur_dma_launch(int64_t m, int64_t n, int nslices, int kslice, const void *Lq, int64_t ldl,
                             const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
-                            const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, hipStream_t s)
+                            const int32_t *eU, int64_t ldeu, int tb, void *C, int64_t ldc, hipStream_t s,
+                            bool c16 = false, float cscale = 1.0f)
 {
     using F = SchurFmt<FMT>;
     const int64_t tiles_m = ceil_div(m, kDmaTile), tiles_n = ceil_div(n, kDmaTile);
@@ -617,18 +674,22 @@ static int schur_dma_launch(int64_t m, int64_t n, int nslices, int kslice, const
     const uint8_t *L8 = static_cast<const uint8_t *>(Lq);
     const uint8_t *U8 = static_cast<const uint8_t *>(Uq);
     const bool whole = vec && m % kDmaTile == 0 && n % kDmaTile == 0;
-    if (whole)
+    if (c16)
+        hipLaunchKernelGGL((schur_dma_kernel<FMT, 3, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
+                           L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
+                           static_cast<int>(tiles_m), static_cast<int>(tiles_n), cscale);
+    else if (whole)
         hipLaunchKernelGGL((schur_dma_kernel<FMT, 2, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
                            L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
-                           static_cast<int>(tiles_m), static_cast<int>(tiles_n));
+                           static_cast<int>(tiles_m), static_cast<int>(tiles_n), 1.0f);
     else if (vec)
         hipLaunchKernelGGL((schur_dma_kernel<FMT, 1, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
                            L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
-                           static_cast<int>(tiles_m), static_cast<int>(tiles_n));
+                           static_cast<int>(tiles_m), static_cast<int>(tiles_n), 1.0f);
     else
         hipLaunchKernelGGL((schur_dma_kernel<FMT, 0, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices, kslice,
                            L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
-                           static_cast<int>(tiles_m), static_cast<int>(tiles_n));
+                           static_cast<int>(tiles_m), static_cast<int>(tiles_n), 1.0f);
     HPLAI_HIP(hipGetLastError());
     return HPLAI_OK;
 }
@@ -636,7 +697,8 @@ static int schur_dma_launch(int64_t m, int64_t n, int nslices, int kslice, const
 template <int FMT, int BM, int BN, bool PACKED>
 static int schur_launch(int64_t m, int64_t n, int nslices, int kslice, const void *Lq, int64_t ldl,
                         const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
-                        const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, hipStream_t s)
+                        const int32_t *eU, int64_t ldeu, int tb, void *C, int64_t ldc, hipStream_t s,
+                        bool c16 = false, float cscale = 1.0f)
 {
     using F = SchurFmt<FMT>;
     const int64_t tiles_m = ceil_div(m, BM), tiles_n = ceil_div(n, BN);
@@ -650,18 +712,22 @@ static int schur_launch(int64_t m, int64_t n, int nslices, int kslice, const voi
     const uint8_t *L8 = static_cast<const uint8_t *>(Lq);
     const uint8_t *U8 = static_cast<const uint8_t *>(Uq);
     const bool whole = vec && m % BM == 0 && n % BN == 0;
-    if (whole)
+    if (c16)
+        hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, 3, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices,
+                           kslice, L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
+                           static_cast<int>(tiles_m), static_cast<int>(tiles_n), cscale);
+    else if (whole)
         hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, 2, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices,
                            kslice, L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
-                           static_cast<int>(tiles_m), static_cast<int>(tiles_n));
+                           static_cast<int>(tiles_m), static_cast<int>(tiles_n), 1.0f);
     else if (vec)
         hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, 1, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices,
                            kslice, L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
-                           static_cast<int>(tiles_m), static_cast<int>(tiles_n));
+                           static_cast<int>(tiles_m), static_cast<int>(tiles_n), 1.0f);
     else
         hipLaunchKernelGGL((schur_kernel<FMT, BM, BN, 0, PACKED>), grid, dim3(kThreads), 0, s, m, n, nslices,
                            kslice, L8, ldl * F::es, eL, ldel, U8, ldu * F::es, eU, ldeu, tb, C, ldc,
-                           static_cast<int>(tiles_m), static_cast<int>(tiles_n));
+                           static_cast<int>(tiles_m), static_cast<int>(tiles_n), 1.0f);
     HPLAI_HIP(hipGetLastError());
     return HPLAI_OK;
 }
@@ -682,23 +748,24 @@ bool schur_large_tiles(int fmt, int64_t m, int64_t n, int tb, int num_cu)
 template <int FMT, bool PACKED>
 static int schur_dispatch(int64_t m, int64_t n, int nslices, int kslice, const void *Lq, int64_t ldl,
                           const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
-                          const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, int num_cu,
-                          hipStream_t s)
+                          const int32_t *eU, int64_t ldeu, int tb, void *C, int64_t ldc, int num_cu,
+                          hipStream_t s, bool c16 = false, float cscale = 1.0f)
 {
     if constexpr (FMT != HPLAI_FMT_F16_TILE) {
-        if (schur_large_tiles(FMT, m, n, tb, num_cu))
+        // the FP16 trailing storage takes the streamed kernel only for whole 256 x 256 tiles
+        if (schur_large_tiles(FMT, m, n, tb, num_cu) && (!c16 || (m % kDmaTile == 0 && n % kDmaTile == 0)))
             return schur_dma_launch<FMT, PACKED>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu,
-                                         tb, C, ldc, s);
+                                         tb, C, ldc, s, c16, cscale);
     }
     return schur_launch<FMT, 128, 128, PACKED>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu, tb,
-                                       C, ldc, s);
+                                       C, ldc, s, c16, cscale);
 }
 
 // Validation lives here so the C ABI and the solver share it.
-int launch_schur(int fmt, int64_t m, int64_t n, int nslices, int kslice, const void *Lq,
-                 int64_t ldl, const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
-                 const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, int num_cu,
-                 hipStream_t s, bool packed)
+static int schur_update(int fmt, int64_t m, int64_t n, int nslices, int kslice, const void *Lq,
+                        int64_t ldl, const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
+                        const int32_t *eU, int64_t ldeu, int tb, void *C, int64_t ldc, int num_cu,
+                        hipStream_t s, bool packed, bool c16, float cscale)
 {
     if (m <= 0 || n <= 0 || nslices <= 0) return HPLAI_OK;
     if (num_cu <= 0) num_cu = 256;
@@ -723,23 +790,47 @@ int launch_schur(int fmt, int64_t m, int64_t n, int nslices, int kslice, const v
     if (packed)
         HPLAI_CHECK((ldl * es) % 64 == 0 && (ldu * es) % 64 == 0,
                     "schur_update: packed operands need rows of whole 64-byte stages");
+    if (c16)
+        HPLAI_CHECK(m % 128 == 0 && n % 128 == 0 && ldc % 4 == 0 && reinterpret_cast<uintptr_t>(C) % 8 == 0 &&
+                        fmt != HPLAI_FMT_F16_TILE,
+                    "schur_update: FP16 trailing storage needs whole 128 x 128 tiles, ldc %% 4 == 0, an "
+                    "8-byte aligned C and the FP16 or e4m3 format");
     switch (fmt) {
     case HPLAI_FMT_F16:
         return packed ? schur_dispatch<HPLAI_FMT_F16, true>(m, n, nslices, kslice, Lq, ldl, nullptr, 0, Uq, ldu,
-                                                            nullptr, 0, 0, C, ldc, num_cu, s)
+                                                            nullptr, 0, 0, C, ldc, num_cu, s, c16, cscale)
                       : schur_dispatch<HPLAI_FMT_F16, false>(m, n, nslices, kslice, Lq, ldl, nullptr, 0, Uq, ldu,
-                                                             nullptr, 0, 0, C, ldc, num_cu, s);
+                                                             nullptr, 0, 0, C, ldc, num_cu, s, c16, cscale);
     case HPLAI_FMT_F16_TILE:
         return packed ? schur_dispatch<HPLAI_FMT_F16_TILE, true>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq,
-                                                                 ldu, eU, ldeu, tb, C, ldc, num_cu, s)
+                                                                 ldu, eU, ldeu, tb, C, ldc, num_cu, s, c16, cscale)
                       : schur_dispatch<HPLAI_FMT_F16_TILE, false>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq,
-                                                                  ldu, eU, ldeu, tb, C, ldc, num_cu, s);
+                                                                  ldu, eU, ldeu, tb, C, ldc, num_cu, s, c16, cscale);
     default:
         return packed ? schur_dispatch<HPLAI_FMT_E4M3, true>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu,
-                                                             eU, ldeu, tb, C, ldc, num_cu, s)
+                                                             eU, ldeu, tb, C, ldc, num_cu, s, c16, cscale)
                       : schur_dispatch<HPLAI_FMT_E4M3, false>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu,
-                                                              eU, ldeu, tb, C, ldc, num_cu, s);
+                                                              eU, ldeu, tb, C, ldc, num_cu, s, c16, cscale);
     }
+}
+
+
+int launch_schur(int fmt, int64_t m, int64_t n, int nslices, int kslice, const void *Lq,
+                 int64_t ldl, const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
+                 const int32_t *eU, int64_t ldeu, int tb, float *C, int64_t ldc, int num_cu,
+                 hipStream_t s, bool packed)
+{
+    return schur_update(fmt, m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu, tb, C, ldc, num_cu, s,
+                        packed, false, 1.0f);
+}
+
+int launch_schur_c16(int fmt, int64_t m, int64_t n, int nslices, int kslice, const void *Lq,
+                     int64_t ldl, const int32_t *eL, int64_t ldel, const void *Uq, int64_t ldu,
+                     const int32_t *eU, int64_t ldeu, int tb, uint16_t *C, int64_t ldc, float cscale,
+                     int num_cu, hipStream_t s, bool packed)
+{
+    return schur_update(fmt, m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu, tb, C, ldc, num_cu, s,
+                        packed, true, cscale);
 }
 
 }  // namespace hplai

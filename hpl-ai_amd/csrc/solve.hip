@@ -50,6 +50,7 @@ struct Buffers {
     double *chk = nullptr;
     float *inv = nullptr;   // diagonal sub-block inverses, one set per diagonal block
     double *tsc = nullptr;  // one block of the IR's triangular solves (scratch)
+    uint16_t *P = nullptr;  // FP16 trailing storage (n x n, leading dimension n)
 };
 
 void free_buffers(Buffers &B, bool keep_factors)
@@ -70,6 +71,7 @@ void free_buffers(Buffers &B, bool keep_factors)
     f(B.info);
     f(B.chk);
     f(B.inv);
+    f(B.P);
     for (int i = 0; i < 2; ++i) {
         f(B.Lq[i]);
         f(B.Uq[i]);
@@ -243,6 +245,9 @@ struct LU {
     bool lazy;
     bool higham = false;          // Higham's matrix instead of the LCG matrix
     bool packed = false;          // Lq / Uq in the panel-packed layout (b % 256 == 0)
+    bool trail16 = false;         // trailing updates into the FP16 storage B->P (lazy only)
+    float pscale = 1.0f;          // B->P holds pscale times the partial sums
+    float pdown = 1.0f;           // (float)(1 / pscale), the lazy initialisation's scale
     double h_alpha = 0.0, h_beta = 0.0;
     uint64_t seed;
     int64_t ntiles;
@@ -282,10 +287,23 @@ struct LU {
     }
     const float *dinv(int64_t c) const { return B->inv + (c / b) * diag_inv_floats(b); }
 
+    // trailing = true: an outer step's update of blocks that have not joined a
+    // panel yet, which goes to the FP16 trailing storage when it is on
     int gemm(int buf, int t0, int nsl, int64_t row0, int64_t col0, int64_t m, int64_t nn,
-             hipStream_t s) const
+             hipStream_t s, bool trailing = false) const
     {
         if (m <= 0 || nn <= 0) return HPLAI_OK;
+        if (trailing && trail16) {
+            const double K = static_cast<double>(nsl) * b;
+            timer->begin(s, 2.0 * static_cast<double>(m) * static_cast<double>(nn) * K,
+                         static_cast<double>(m + nn) * K * es + 4.0 * static_cast<double>(m) * nn,
+                         schur_large_tiles(fmt, m, nn, b, num_cu), false);
+            const int rc = launch_schur_c16(fmt, m, nn, nsl, kpad, lq(buf, row0, t0), ldq, el(buf, t0, row0),
+                                            ntiles, uq(buf, col0, t0), ldq, eu(buf, t0, col0), ntiles, b,
+                                            B->P + col0 * n + row0, n, pscale, num_cu, s, packed);
+            timer->end(s);
+            return rc;
+        }
         const double K = static_cast<double>(nsl) * b;
         timer->begin(s, 2.0 * static_cast<double>(m) * static_cast<double>(nn) * K,
                      static_cast<double>(m + nn) * K * es + 8.0 * static_cast<double>(m) * nn,
@@ -314,6 +332,9 @@ struct LU {
         if (!lazy || m <= 0 || k <= 0) return HPLAI_OK;
         return timed(HPLAI_PHASE_LAZY_INIT, static_cast<double>(m) * k, s, [&] {
             if (higham) return launch_gen_higham(h_alpha, h_beta, i0, j0, m, k, a(i0, j0), lda, 1, s);
+            if (trail16)   // A += a_ij + P / pscale (the partial sums of earlier outer steps)
+                return launch_gen_matrix(n, seed, i0, j0, m, k, B->d, a(i0, j0), lda, 1, s, B->P + j0 * n + i0,
+                                         n, pdown);
             return launch_gen_matrix(n, seed, i0, j0, m, k, B->d, a(i0, j0), lda, 1, s);
         });
     }
@@ -479,7 +500,7 @@ struct LU {
                     // next panel's columns first, then hand them to the side stream
                     const int64_t B1 = static_cast<int64_t>(sw1) * b;
                     const bool ahead = lookahead > 1 && rest > B1;
-                    rc = gemm(buf, 0, sw, r0, r0, rest, B1, main);
+                    rc = gemm(buf, 0, sw, r0, r0, rest, B1, main, true);
                     if (rc) break;
                     HPLAI_HIP(hipEventRecord(ev[0], main));
                     HPLAI_HIP(hipStreamWaitEvent(side, ev[0], 0));
@@ -488,25 +509,25 @@ struct LU {
                     rc = panel(c1, sw1, buf ^ 1, side, !ahead);
                     if (rc) break;
                     if (ahead) {
-                        rc = gemm(buf, 0, sw, r0, r0 + B1, B1, rest - B1, main);
+                        rc = gemm(buf, 0, sw, r0, r0 + B1, B1, rest - B1, main, true);
                         if (rc) break;
                         HPLAI_HIP(hipEventRecord(ev[2], main));
                         HPLAI_HIP(hipStreamWaitEvent(side, ev[2], 0));
                         rc = block_row(c1, sw1, buf ^ 1, side);
                         if (rc) break;
                         HPLAI_HIP(hipEventRecord(ev[1], side));
-                        rc = gemm(buf, 0, sw, r0 + B1, r0 + B1, rest - B1, rest - B1, main);
+                        rc = gemm(buf, 0, sw, r0 + B1, r0 + B1, rest - B1, rest - B1, main, true);
                         if (rc) break;
                         HPLAI_HIP(hipStreamWaitEvent(main, ev[6], 0));   // the panel's bulk stream
                         row_done = true;
                     } else {
                         HPLAI_HIP(hipEventRecord(ev[1], side));
-                        rc = gemm(buf, 0, sw, r0, r0 + B1, rest, rest - B1, main);
+                        rc = gemm(buf, 0, sw, r0, r0 + B1, rest, rest - B1, main, true);
                         if (rc) break;
                     }
                     HPLAI_HIP(hipStreamWaitEvent(main, ev[1], 0));
                 } else {
-                    rc = gemm(buf, 0, sw, r0, r0, rest, rest, main);
+                    rc = gemm(buf, 0, sw, r0, r0, rest, rest, main, true);
                     if (rc == HPLAI_OK) rc = panel(c1, sw1, buf ^ 1, main);
                     if (rc) break;
                 }
@@ -716,6 +737,20 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
             lu.h_beta = -lu.h_beta;
         }
     }
+    if (p->trail16) {
+        // 16-bit trailing storage (the reference's -d double-decker lower deck):
+        // partial sums of not yet factored blocks in FP16, scaled by
+        // scalea = sqrt(n sqrt(n)) (hpl_rand.hpp:154) and merged by the lazy
+        // initialisation with downscale = 1 / scalea (panel_trf.hpp:343-345)
+        HPLAI_CHECK(p->lazy != 0, "hplai_solve: the FP16 trailing storage needs lazy initialisation");
+        HPLAI_CHECK(p->matrix == HPLAI_MATRIX_HPLAI, "hplai_solve: the FP16 trailing storage takes the HPL-AI matrix");
+        HPLAI_CHECK(fmt != HPLAI_FMT_F16_TILE && b % 128 == 0,
+                    "hplai_solve: the FP16 trailing storage needs b %% 128 == 0 and the FP16 or e4m3 format");
+        lu.trail16 = true;
+        const double sa = std::sqrt(static_cast<double>(n) * std::sqrt(static_cast<double>(n)));
+        lu.pscale = static_cast<float>(sa);
+        lu.pdown = static_cast<float>(1.0 / static_cast<double>(lu.pscale));
+    }
     EpochLog epochs;
     epochs.on = p->verbose != 0;
     epochs.n = n;
@@ -741,6 +776,7 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     ALLOC(B.work, sizeof(double) * std::max(otf_gemv_work_doubles(n), higham_gemv_work_doubles(n)));
     ALLOC(B.norms, sizeof(double) * 4);
     ALLOC(B.chk, sizeof(double) * 3);
+    if (lu.trail16) ALLOC(B.P, sizeof(uint16_t) * n * n);
 #undef ALLOC
     rc = alloc_lu_buffers(lu, B, s);
     if (rc) {
@@ -757,6 +793,7 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     if (lu.lazy) {
         // pmatgen0 (matgen.hpp:80-105): partial sums start at zero
         HPLAI_HIP(hipMemsetAsync(B.A, 0, sizeof(float) * lu.lda * n, s));
+        if (lu.trail16) HPLAI_HIP(hipMemsetAsync(B.P, 0, sizeof(uint16_t) * n * n, s));
     } else if (rc == HPLAI_OK) {
         rc = lu.higham ? launch_gen_higham(lu.h_alpha, lu.h_beta, 0, 0, n, n, B.A, lu.lda, 0, s)
                        : launch_gen_matrix(n, p->seed, 0, 0, n, n, B.d, B.A, lu.lda, 0, s);

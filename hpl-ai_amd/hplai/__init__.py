@@ -40,7 +40,8 @@ class HplaiError(RuntimeError):
 class Params(ctypes.Structure):
     _fields_ = [("n", _i64), ("b", _int), ("seed", _u64), ("fmt", _int), ("outer", _int),
                 ("lookahead", _int), ("lazy", _int), ("maxit", _int), ("verbose", _int),
-                ("keep_factors", _int), ("profile", _int), ("matrix", _int), ("cond", _dbl)]
+                ("keep_factors", _int), ("profile", _int), ("matrix", _int), ("cond", _dbl),
+                ("trail16", _int)]
 
 
 class Result(ctypes.Structure):
@@ -78,6 +79,8 @@ _SIGS = [
     ("hplai_quantize", _int, [_vp, _int, _int, _i64, _i64, _int, _vp, _i64, _vp, _i64, _vp, _vp]),
     ("hplai_schur_update", _int, [_vp, _int, _i64, _i64, _int, _int, _vp, _i64, _vp, _i64, _vp, _i64,
                                   _vp, _i64, _int, _vp, _i64, _vp]),
+    ("hplai_schur_update_f16c", _int, [_vp, _int, _i64, _i64, _int, _int, _vp, _i64, _vp, _i64, _vp, _i64,
+                                       _vp, _i64, _int, _vp, _i64, ctypes.c_float, _vp]),
     ("hplai_otf_gemv_f64", _int, [_vp, _i64, _u64, _vp, _dbl, _vp, _vp, _vp]),
     ("hplai_higham_coeffs", _int, [_i64, _dbl, _dbl, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]),
     ("hplai_gen_higham_f32", _int, [_vp, _i64, _dbl, _dbl, _i64, _i64, _i64, _i64, _vp, _i64, _vp]),
@@ -214,6 +217,13 @@ class Context:
                                        ldel, ptr(Uq), ldu, ptr(eU), ldeu, tb, ptr(C), ldc,
                                        stream_ptr(stream)))
 
+    def schur_update_f16c(self, fmt, m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu, tb, C,
+                          ldc, cscale, stream=None):
+        """C (FP16 bits, m x n column-major) := fp16(C - cscale * L U)."""
+        check(lib().hplai_schur_update_f16c(self.handle, fmt, m, n, nslices, kslice, ptr(Lq), ldl, ptr(eL),
+                                            ldel, ptr(Uq), ldu, ptr(eU), ldeu, tb, ptr(C), ldc, cscale,
+                                            stream_ptr(stream)))
+
     # ---- IR ----
     # ---- Higham's HPL-AI matrix ----
     def gen_higham(self, n, alpha, beta, i0, j0, m, k, A, lda, stream=None):
@@ -250,15 +260,16 @@ class Context:
 
     # ---- whole solve ----
     def solve(self, n, b, fmt="f16", outer=1, lookahead=True, lazy=False, seed=42, maxit=50,
-              verbose=False, keep_factors=False, profile=False, matrix="hplai", cond=0.0):
+              verbose=False, keep_factors=False, profile=False, matrix="hplai", cond=0.0, trail16=False):
         """matrix: "hplai" (the reference's LCG matrix), "higham" (--highammat as
         the reference generates it) or "higham_kappa" (Fasi and Higham's signs,
-        cond(A, inf) = cond exactly); cond 0 = 1000."""
+        cond(A, inf) = cond exactly); cond 0 = 1000.  trail16 (with lazy): the
+        partial sums of blocks not yet factored kept in FP16."""
         p = Params(n=n, b=b, seed=seed, fmt=FMTS[fmt] if isinstance(fmt, str) else fmt, outer=outer,
                    lookahead=int(lookahead), lazy=1 if lazy else 0, maxit=maxit,
                    verbose=1 if verbose else 0, keep_factors=1 if keep_factors else 0,
                    profile=1 if profile else 0, matrix=MATRICES[matrix] if isinstance(matrix, str) else matrix,
-                   cond=cond)
+                   cond=cond, trail16=1 if trail16 else 0)
         r = Result()
         rc = lib().hplai_solve(self.handle, ctypes.byref(p), ctypes.byref(r))
         if rc not in (OK, ENOCONV):

@@ -103,6 +103,16 @@ int hplai_schur_update(hplai_ctx ctx, int fmt, int64_t m, int64_t n, int nslices
                        const void *Lq, int64_t ldl, const int32_t *eL, int64_t ldel,
                        const void *Uq, int64_t ldu, const int32_t *eU, int64_t ldeu, int tb,
                        float *C, int64_t ldc, void *stream);
+/* Replaces the FP16-C overload gemmschur(fp16 a, fp16 b, fp16 c) (schur_updator.hpp:24-27,
+ * hgemm with alpha = -1): C := fp16(C - cscale * 2^-(eL+eU) Lq Uq) with C an FP16
+ * (binary16 bits) column-major m x n array, FP32 accumulation (the reference rounds
+ * its accumulator to FP16 after every product).  m, n multiples of 128, ldc % 4 == 0,
+ * C 8-byte aligned; fmt HPLAI_FMT_F16 or HPLAI_FMT_E4M3; other arguments as in
+ * hplai_schur_update. */
+int hplai_schur_update_f16c(hplai_ctx ctx, int fmt, int64_t m, int64_t n, int nslices, int kslice,
+                            const void *Lq, int64_t ldl, const int32_t *eL, int64_t ldel,
+                            const void *Uq, int64_t ldu, const int32_t *eU, int64_t ldeu, int tb,
+                            uint16_t *C, int64_t ldc, float cscale, void *stream);
 
 /* ---------------- Higham's HPL-AI matrix (--highammat) ---------------- */
 /* Replaces HMGen (highammgen.hpp:5-27) with higham_mat_comp_beta
@@ -216,6 +226,10 @@ typedef struct hplai_params {
     int      profile;    /* time every Schur-update launch with HIP events         */
     int      matrix;     /* HPLAI_MATRIX_*: the input matrix family                */
     double   cond;       /* Higham matrix: cond(A, inf); 0 = 1000 as in main.cpp:258 */
+    int      trail16;    /* 1 (with lazy = 1): the trailing updates keep the partial sums of
+                            blocks not yet factored in an FP16 n x n array, scaled by
+                            sqrt(n sqrt(n)) (the reference's -d double-decker lower deck,
+                            panel.hpp:45-83, panel_trf.hpp:343-345); b % 128 == 0           */
 } hplai_params;
 
 #define HPLAI_MATRIX_HPLAI        0   /* the reference's LCG matrix with diagonal row sums */

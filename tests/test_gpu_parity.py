@@ -264,6 +264,59 @@ def test_schur_update_vs_fp64(ctx, oracle, fmt, m, n, nsl, tb):
     assert not bad.any(), "max err/tol %.3g at %s" % ((err / tol).max(), np.argwhere(bad)[:3])
 
 
+@pytest.mark.parametrize("fmt,m,n,nsl,tb", [("f16", 1024, 768, 2, 128), ("e4m3", 768, 1024, 2, 256),
+                                            # >= 512 tiles of 256 x 256: the streamed kernel
+                                            ("e4m3", 6144, 6400, 2, 256), ("f16", 6400, 6144, 2, 256)])
+def test_schur_update_f16c_vs_fp64(ctx, oracle, fmt, m, n, nsl, tb):
+    """The FP16 trailing storage: C := fp16(C - cscale L U), FP32 accumulation.
+    Bound: the FP32 product bound of the FP32-C test times cscale, plus one
+    rounding to binary16 (2^-11 relative, 2^-24 absolute for subnormals)."""
+    import hplai
+    rng = np.random.default_rng(12)
+    kslice = 128
+    K = nsl * kslice
+    ntm, ntn = -(-m // tb), -(-n // tb)
+    cscale = 37.25
+    if fmt == "e4m3":
+        Lbits = rng.integers(0, 256, size=(m, K), dtype=np.uint8)
+        Ubits = rng.integers(0, 256, size=(n, K), dtype=np.uint8)
+        Lbits[(Lbits & 0x7f) == 0x7f] = 0x3c
+        Ubits[(Ubits & 0x7f) == 0x7f] = 0x3c
+        Lv = oracle.e4m3_to_f32(Lbits).astype(np.float64)
+        Uv = oracle.e4m3_to_f32(Ubits).astype(np.float64)
+        # scales that keep cscale |L U| (<= 448^2 K 2^-18 cscale ~ 1.5e4) inside binary16
+        eL = rng.integers(9, 14, size=(nsl, ntm)).astype(np.int32)
+        eU = rng.integers(9, 14, size=(nsl, ntn)).astype(np.int32)
+    else:
+        Lv16 = (rng.standard_normal((m, K)) * 0.05).astype(np.float16)
+        Uv16 = (rng.standard_normal((n, K)) * 0.05).astype(np.float16)
+        Lbits, Ubits = Lv16.view(np.uint8), Uv16.view(np.uint8)
+        Lv, Uv = Lv16.astype(np.float64), Uv16.astype(np.float64)
+        eL = np.zeros((nsl, ntm), dtype=np.int32)
+        eU = np.zeros((nsl, ntn), dtype=np.int32)
+    C0 = (rng.standard_normal((n, m)) * 8).astype(np.float16)   # C[col, row], ldc = m
+    gC = _t(C0.view(np.uint16))
+    ctx.schur_update_f16c(hplai.FMTS[fmt], m, n, nsl, kslice, _t(Lbits.reshape(-1)), K,
+                          None if fmt == "f16" else _t(eL), ntm, _t(Ubits.reshape(-1)), K,
+                          None if fmt == "f16" else _t(eU), ntn, tb, gC, m, cscale)
+    _sync()
+    got = gC.cpu().numpy().view(np.float16).astype(np.float64)
+    P = np.zeros((m, n))
+    Pabs = np.zeros((m, n))
+    ri, ci = np.arange(m) // tb, np.arange(n) // tb
+    for sl in range(nsl):
+        Ls, Us = Lv[:, sl * kslice:(sl + 1) * kslice], Uv[:, sl * kslice:(sl + 1) * kslice]
+        sc = np.ldexp(1.0, -(eL[sl][ri][:, None] + eU[sl][ci][None, :]))
+        P += sc * (Ls @ Us.T)
+        Pabs += sc * (np.abs(Ls) @ np.abs(Us.T))
+    want = C0.T.astype(np.float64) - cscale * P
+    tolp = 4 * K * 2.0 ** -24 * Pabs + (2.0 ** -10 * Pabs if fmt == "e4m3" else 0.0)
+    tol = cscale * tolp + 2.0 ** -23 * np.abs(C0.T.astype(np.float64)) + 2.0 ** -11 * np.abs(want) + 2.0 ** -24
+    err = np.abs(got.T - want)
+    bad = err > tol
+    assert not bad.any(), "max err/tol %.3g at %s" % ((err / tol).max(), np.argwhere(bad)[:3])
+
+
 # ------------------------------------------------------------------ IR kernels
 @pytest.mark.parametrize("n", [1200, 5000])
 def test_otf_gemv_vs_oracle(ctx, oracle, n):
@@ -387,6 +440,33 @@ def test_solve_packed_operands_match_oracle(ctx, oracle, fmt, outer, mode, tol, 
     rel = np.max(np.abs(lu - ref)) / np.max(np.abs(ref))
     print(fmt, "factors vs oracle: %.3g" % rel)
     assert rel < tol, rel
+
+
+@pytest.mark.parametrize("fmt,outer,lookahead", [("e4m3", 2, 1), ("f16", 2, 2), ("e4m3", 1, 0)])
+def test_solve_trail16(ctx, oracle, fmt, outer, lookahead):
+    """16-bit trailing storage (lazy initialisation, partial sums of blocks not
+    yet factored in FP16 scaled by sqrt(n sqrt(n))).  The factors differ from the
+    oracle's FP32-partial-sum LU of the same format by the FP16 roundings of the
+    partial sums (at most one per outer step and element); measured ~1e-5
+    relative, bound 2e-4.  The solve still reaches the harness."""
+    n, b = 8192, 512
+    res = ctx.solve(n, b, fmt=fmt, outer=outer, lookahead=lookahead, lazy=True, trail16=True, keep_factors=True)
+    lu = ctx.solution_factors(n)
+    ctx.release()
+    assert res["converged"] and res["harness"] < 16.0, res
+    assert res["ir_steps"] <= 6
+    ref, d = oracle.lu(oracle.LU_FP8 if fmt == "e4m3" else oracle.LU_PRECOMP, n, b, SEED)
+    rel = np.max(np.abs(lu - ref)) / np.max(np.abs(ref))
+    print(fmt, outer, "trail16 factors vs oracle: %.3g" % rel)
+    assert rel < 2e-4, rel
+
+
+def test_solve_trail16_rejects_bad_options(ctx):
+    import hplai
+    with pytest.raises(hplai.HplaiError):
+        ctx.solve(2048, 512, fmt="e4m3", lazy=False, trail16=True)
+    with pytest.raises(hplai.HplaiError):
+        ctx.solve(2048, 512, fmt="f16t", lazy=True, trail16=True)
 
 
 def test_solve_c1_lazy(ctx):
