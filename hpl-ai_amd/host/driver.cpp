@@ -27,7 +27,11 @@ static void usage()
         "   P                 Process-grid rows of the reference; reported only.\n"
         "   -not, -nt         Accepted (the layout is always one column-major matrix).\n"
         "   -d, -l            Lazy initialisation: the trailing matrix starts at zero and\n"
-        "                     each block is regenerated and merged when it joins a panel.\n"
+        "                     each block is regenerated and merged when it joins a panel;\n"
+        "                     the partial sums of blocks not yet factored are kept in FP16\n"
+        "                     (the reference's double-decker lower deck) where b %% 128 == 0\n"
+        "                     and the format is f16 or e4m3, in FP32 otherwise.\n"
+        "   --trail32         With -d: keep the partial sums in FP32.\n"
         "   -r, --rdma        Accepted (no inter-process broadcast on one device).\n"
         "   -w, --warmup      One IR step only.\n"
         "   --checksum        Print the factor checksum line.\n"
@@ -101,7 +105,7 @@ int main(int argc, char **argv)
     p.profile = 1;   // per-phase times for the Timerdump file
     int jobid = 0;
     bool checksum = false, warmup = false, lazy = false, rdma = false, tile = true, higham = false;
-    bool higham_kappa = false;
+    bool higham_kappa = false, trail32 = false;
     int device = 0;
     for (int i = 4; i < argc; ++i) {
         const char *a = argv[i];
@@ -112,6 +116,7 @@ int main(int argc, char **argv)
         else if (!std::strcmp(a, "-r") || !std::strcmp(a, "--rdma")) rdma = true;
         else if (!std::strcmp(a, "-w") || !std::strcmp(a, "--warmup")) warmup = true;
         else if (!std::strcmp(a, "--checksum")) checksum = true;
+        else if (!std::strcmp(a, "--trail32")) trail32 = true;
         else if (!std::strcmp(a, "--highammat")) higham = true;
         else if (!std::strcmp(a, "--highammat-kappa")) higham = higham_kappa = true;
         else if (!std::strcmp(a, "--fmt")) {
@@ -133,6 +138,8 @@ int main(int argc, char **argv)
     if (warmup) p.maxit = 1;
     p.lazy = lazy ? 1 : 0;
     p.matrix = higham_kappa ? HPLAI_MATRIX_HIGHAM_KAPPA : (higham ? HPLAI_MATRIX_HIGHAM : HPLAI_MATRIX_HPLAI);
+    // -d is the reference's double-decker layout (main.cpp:52-54): FP16 partial sums
+    p.trail16 = lazy && !trail32 && b % 128 == 0 && p.fmt != HPLAI_FMT_F16_TILE && !higham ? 1 : 0;
     p.cond = 1000.0;
 
     std::printf("jobid=%d\n", jobid);
@@ -140,9 +147,9 @@ int main(int argc, char **argv)
     std::printf("%s %s %s %s nopack %s noskiplu %s\n", tile ? "tile" : "2dbc",
                 lazy ? "lazy" : "precomp", rdma ? "rdma" : "mpi", warmup ? "warmup" : "full",
                 checksum ? "checksum" : "nocheck", higham ? "hmat" : "ddmat");
-    std::printf("device=%d P=%d fmt=%s outer=%d lookahead=%d\n", device, nrow,
+    std::printf("device=%d P=%d fmt=%s outer=%d lookahead=%d trailing=%s\n", device, nrow,
                 p.fmt == HPLAI_FMT_F16 ? "f16" : (p.fmt == HPLAI_FMT_F16_TILE ? "f16t" : "e4m3"),
-                p.outer, p.lookahead);
+                p.outer, p.lookahead, p.trail16 ? "fp16" : "fp32");
     std::printf("epoch_size = %d\n", b);   // b * lcm(1, 1) on a 1 x 1 grid (main.cpp:193)
     print_ctime("#BEGIN: ");
     std::fflush(stdout);

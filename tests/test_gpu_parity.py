@@ -511,3 +511,21 @@ def test_driver_output_and_timerdump(tmp_path):
     assert lines[22].startswith("GEMM_TOTAL: ")
     sec = {ln.split(":")[0].strip(): float(ln.split(":")[1].split()[0]) for ln in lines[1:22]}
     assert sec["TOTAL"] > 0 and sec["DIAG_LU"] > 0 and sec["GEMM_UPDATE"] > 0 and sec["IR_TRSV"] > 0
+
+
+@pytest.mark.parametrize("extra,trailing", [([], "fp16"), (["--trail32"], "fp32")])
+def test_driver_delayed_trailing_storage(tmp_path, extra, trailing):
+    """driver.out -d with b % 128 == 0: the reference's double-decker layout,
+    partial sums in FP16 (or FP32 with --trail32); both reach the harness."""
+    import os
+    import re
+    import subprocess
+    import hplai
+    if not os.path.exists(hplai.DRIVER_PATH):
+        pytest.skip("driver.out not built")
+    out = subprocess.run([hplai.DRIVER_PATH, "4096", "512", "1", "-d", "--fmt", "e4m3", "--outer", "2",
+                          "--no-timerdump"] + extra, capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr
+    assert ("trailing=%s" % trailing) in out.stdout and "lazy" in out.stdout
+    m = re.search(r"hpl-harness = ([0-9.]+)$", out.stdout, re.M)
+    assert m and float(m.group(1)) < 16.0
