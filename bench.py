@@ -247,7 +247,7 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=196608)
     ap.add_argument("--b", type=int, default=512)
     ap.add_argument("--fmt", default="e4m3", choices=["e4m3", "f16", "f16t"])
