@@ -1,7 +1,8 @@
 // trsm.hip — panel TRSMs of the blocked LU with the whole panel height in LDS.
 //
 // One workgroup owns a strip of 32 columns (U panel) or 32 rows (L panel) of
-// the panel and all b <= 512 entries of its other dimension, in LDS.
+// the panel (64 for the triangular products when b % 64 != 0) and all
+// b <= 512 entries of its other dimension, in LDS.
 //  * b % 32 == 0: the triangles of the factored diagonal block are inverted
 //    in full once per block (tri_inv_kernel, after the 32 x 32 sub-block
 //    inverses of diag_inv32_kernel), and every strip is a triangular matrix
