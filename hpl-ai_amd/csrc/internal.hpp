@@ -46,7 +46,11 @@ __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + 
 // trailing storage's partial sums of the block)
 int launch_gen_matrix(int64_t n, uint64_t seed, int64_t i0, int64_t j0, int64_t m, int64_t k,
                       const double *d, float *A, int64_t lda, int accumulate, hipStream_t s,
-                      const uint16_t *P = nullptr, int64_t ldp = 0, float pscale = 0.0f);
+                      const uint16_t *P = nullptr, int64_t ldp = 0, float pscale = 0.0f,
+                      const uint16_t *Q = nullptr);
+// back-buffer write-back of an m x k block of the FP16 trailing storage:
+// Q := fl16(Q + P / 32), P := the rounding error times 32
+int launch_backbuffer(int64_t m, int64_t k, uint16_t *P, uint16_t *Q, int64_t ldp, hipStream_t s);
 int launch_gen_diag(int64_t n, uint64_t seed, int64_t i0, int64_t cnt, double *d, hipStream_t s);
 // Higham's HPL-AI matrix (higham.hip)
 int higham_coeffs(int64_t n, double cond, double rho, double *alpha, double *beta);

@@ -17,7 +17,7 @@ constexpr int kGenCols = 64;    // columns per thread: amortises the jump-ahead
 __global__ __launch_bounds__(kGenRows) void gen_matrix_kernel(
     int64_t n, uint64_t x1, int64_t i0, int64_t j0, int64_t m, int64_t k, int64_t row_blocks,
     LcgCoeff jn, const double *__restrict__ d, float *__restrict__ A, int64_t lda, int accumulate,
-    const uint16_t *__restrict__ P, int64_t ldp, float pscale)
+    const uint16_t *__restrict__ P, const uint16_t *__restrict__ Q, int64_t ldp, float pscale)
 {
     const int64_t rb = blockIdx.x % row_blocks;
     const int64_t cg = blockIdx.x / row_blocks;
@@ -30,6 +30,7 @@ __global__ __launch_bounds__(kGenRows) void gen_matrix_kernel(
                                   static_cast<uint64_t>(n) * static_cast<uint64_t>(j0 + jl0));
     float *col = A + jl0 * lda + il;
     const uint16_t *pcol = P != nullptr ? P + jl0 * ldp + il : nullptr;
+    const uint16_t *qcol = Q != nullptr ? Q + jl0 * ldp + il : nullptr;
     for (int64_t jl = jl0; jl < jl1; ++jl, col += lda) {
         float v = static_cast<float>(lcg_value(s));
         if (d != nullptr && gi == j0 + jl) v = static_cast<float>(d[gi]);
@@ -37,7 +38,13 @@ __global__ __launch_bounds__(kGenRows) void gen_matrix_kernel(
         // (LazyInitializer::update, lazy_init.hpp:147-227); the FP16 trailing
         // storage holds the partial sums of earlier outer steps times 1 / scale
         if (pcol != nullptr) {
-            v = fmaf(pscale, f16_bits_to_f32(*pcol), v);
+            // with the back buffer Q (lazy_init.hpp:229-329): x = P + 32 Q
+            float x = f16_bits_to_f32(*pcol);
+            if (qcol != nullptr) {
+                x = fmaf(32.0f, f16_bits_to_f32(*qcol), x);
+                qcol += ldp;
+            }
+            v = fmaf(pscale, x, v);
             pcol += ldp;
         }
         *col = accumulate ? *col + v : v;
@@ -97,7 +104,7 @@ __global__ void gen_rhs_kernel(int64_t n, uint64_t x1, int64_t i0, int64_t cnt,
 
 int launch_gen_matrix(int64_t n, uint64_t seed, int64_t i0, int64_t j0, int64_t m, int64_t k,
                       const double *d, float *A, int64_t lda, int accumulate, hipStream_t s,
-                      const uint16_t *P, int64_t ldp, float pscale)
+                      const uint16_t *P, int64_t ldp, float pscale, const uint16_t *Q)
 {
     if (m <= 0 || k <= 0) return HPLAI_OK;
     const int64_t row_blocks = ceil_div(m, kGenRows);
@@ -111,8 +118,41 @@ int launch_gen_matrix(int64_t n, uint64_t seed, int64_t i0, int64_t j0, int64_t 
         int64_t kk = (k - jj) < g * kGenCols ? (k - jj) : g * kGenCols;
         hipLaunchKernelGGL(gen_matrix_kernel, dim3(static_cast<unsigned>(row_blocks * g)),
                            dim3(kGenRows), 0, s, n, lcg_init(seed), i0, j0 + jj, m, kk, row_blocks,
-                           jn, d, A + jj * lda, lda, accumulate, P != nullptr ? P + jj * ldp : nullptr, ldp,
-                           pscale);
+                           jn, d, A + jj * lda, lda, accumulate, P != nullptr ? P + jj * ldp : nullptr,
+                           Q != nullptr ? Q + jj * ldp : nullptr, ldp, pscale);
+        HPLAI_HIP(hipGetLastError());
+    }
+    return HPLAI_OK;
+}
+
+// Back-buffer write-back (bb_writeback_b_impl_fp16, back_buffer.cpp:32-55 in
+// its non-SVE form): the front P (the trailing updates' target) folds into
+// the back Q with a compensated sum in FP16 arithmetic, Q := fl(Q + P/32),
+// P := 32 (P/32 - (Q_new - Q)), so P keeps the rounding error.
+__global__ __launch_bounds__(256) void backbuffer_kernel(int64_t m, int64_t k, uint16_t *__restrict__ P,
+                                                         uint16_t *__restrict__ Q, int64_t ldp)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= m) return;
+    const int64_t j = blockIdx.y;
+    uint16_t *pp = P + j * ldp + i, *qq = Q + j * ldp + i;
+    const _Float16 t0 = __builtin_bit_cast(_Float16, *qq);
+    const _Float16 t1 = __builtin_bit_cast(_Float16, *pp) * static_cast<_Float16>(1.0f / 32.0f);
+    const _Float16 t2 = t0 + t1;
+    const _Float16 t3 = t2 - t0;
+    const _Float16 t4 = t1 - t3;
+    *qq = __builtin_bit_cast(uint16_t, t2);
+    *pp = __builtin_bit_cast(uint16_t, static_cast<_Float16>(t4 * static_cast<_Float16>(32.0f)));
+    (void)k;
+}
+
+int launch_backbuffer(int64_t m, int64_t k, uint16_t *P, uint16_t *Q, int64_t ldp, hipStream_t s)
+{
+    if (m <= 0 || k <= 0) return HPLAI_OK;
+    for (int64_t j0 = 0; j0 < k; j0 += 65535) {
+        const int64_t kk = k - j0 < 65535 ? k - j0 : 65535;
+        hipLaunchKernelGGL(backbuffer_kernel, dim3(static_cast<unsigned>(ceil_div(m, 256)), static_cast<unsigned>(kk)),
+                           dim3(256), 0, s, m, kk, P + j0 * ldp, Q + j0 * ldp, ldp);
         HPLAI_HIP(hipGetLastError());
     }
     return HPLAI_OK;

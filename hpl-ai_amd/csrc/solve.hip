@@ -51,6 +51,7 @@ struct Buffers {
     float *inv = nullptr;   // diagonal sub-block inverses, one set per diagonal block
     double *tsc = nullptr;  // one block of the IR's triangular solves (scratch)
     uint16_t *P = nullptr;  // FP16 trailing storage (n x n, leading dimension n)
+    uint16_t *Q = nullptr;  // its back buffer (n x n), when on
 };
 
 void free_buffers(Buffers &B, bool keep_factors)
@@ -72,6 +73,7 @@ void free_buffers(Buffers &B, bool keep_factors)
     f(B.chk);
     f(B.inv);
     f(B.P);
+    f(B.Q);
     for (int i = 0; i < 2; ++i) {
         f(B.Lq[i]);
         f(B.Uq[i]);
@@ -248,6 +250,7 @@ struct LU {
     bool trail16 = false;         // trailing updates into the FP16 storage B->P (lazy only)
     float pscale = 1.0f;          // B->P holds pscale times the partial sums
     float pdown = 1.0f;           // (float)(1 / pscale), the lazy initialisation's scale
+    bool bback = false;           // back buffer B->Q of the FP16 trailing storage
     double h_alpha = 0.0, h_beta = 0.0;
     uint64_t seed;
     int64_t ntiles;
@@ -334,7 +337,7 @@ struct LU {
             if (higham) return launch_gen_higham(h_alpha, h_beta, i0, j0, m, k, a(i0, j0), lda, 1, s);
             if (trail16)   // A += a_ij + P / pscale (the partial sums of earlier outer steps)
                 return launch_gen_matrix(n, seed, i0, j0, m, k, B->d, a(i0, j0), lda, 1, s, B->P + j0 * n + i0,
-                                         n, pdown);
+                                         n, pdown, bback ? B->Q + j0 * n + i0 : nullptr);
             return launch_gen_matrix(n, seed, i0, j0, m, k, B->d, a(i0, j0), lda, 1, s);
         });
     }
@@ -531,6 +534,14 @@ struct LU {
                     if (rc == HPLAI_OK) rc = panel(c1, sw1, buf ^ 1, main);
                     if (rc) break;
                 }
+            }
+            // back-buffer write-back every 32 trailing updates (BackBuffer::write_back,
+            // back_buffer.hpp:81-170, BB_NCYCLE = 32), over the blocks that do not
+            // join the next outer block (those merge P and Q at their lazy initialisation)
+            if (bback && (step + 1) % 32 == 0 && rest > 0) {
+                const int64_t r1 = c1 + static_cast<int64_t>(sw1) * b;
+                rc = launch_backbuffer(n - r1, n - r1, B->P + r1 * n + r1, B->Q + r1 * n + r1, n, main);
+                if (rc) break;
             }
             c0 = c1;
             sw = sw1;
@@ -747,6 +758,7 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
         HPLAI_CHECK(fmt != HPLAI_FMT_F16_TILE && b % 128 == 0,
                     "hplai_solve: the FP16 trailing storage needs b %% 128 == 0 and the FP16 or e4m3 format");
         lu.trail16 = true;
+        lu.bback = p->trail16 == 2;
         const double sa = std::sqrt(static_cast<double>(n) * std::sqrt(static_cast<double>(n)));
         lu.pscale = static_cast<float>(sa);
         lu.pdown = static_cast<float>(1.0 / static_cast<double>(lu.pscale));
@@ -777,6 +789,7 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
     ALLOC(B.norms, sizeof(double) * 4);
     ALLOC(B.chk, sizeof(double) * 3);
     if (lu.trail16) ALLOC(B.P, sizeof(uint16_t) * n * n);
+    if (lu.bback) ALLOC(B.Q, sizeof(uint16_t) * n * n);
 #undef ALLOC
     rc = alloc_lu_buffers(lu, B, s);
     if (rc) {
@@ -794,6 +807,7 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
         // pmatgen0 (matgen.hpp:80-105): partial sums start at zero
         HPLAI_HIP(hipMemsetAsync(B.A, 0, sizeof(float) * lu.lda * n, s));
         if (lu.trail16) HPLAI_HIP(hipMemsetAsync(B.P, 0, sizeof(uint16_t) * n * n, s));
+        if (lu.bback) HPLAI_HIP(hipMemsetAsync(B.Q, 0, sizeof(uint16_t) * n * n, s));
     } else if (rc == HPLAI_OK) {
         rc = lu.higham ? launch_gen_higham(lu.h_alpha, lu.h_beta, 0, 0, n, n, B.A, lu.lda, 0, s)
                        : launch_gen_matrix(n, p->seed, 0, 0, n, n, B.d, B.A, lu.lda, 0, s);

@@ -32,6 +32,8 @@ static void usage()
         "                     (the reference's double-decker lower deck) where b %% 128 == 0\n"
         "                     and the format is f16 or e4m3, in FP32 otherwise.\n"
         "   --trail32         With -d: keep the partial sums in FP32.\n"
+        "   --back_buffer     -d with the compensated FP16 back buffer (a second FP16\n"
+        "                     n x n array, folded every 32 trailing updates).\n"
         "   -r, --rdma        Accepted (no inter-process broadcast on one device).\n"
         "   -w, --warmup      One IR step only.\n"
         "   --checksum        Print the factor checksum line.\n"
@@ -105,7 +107,7 @@ int main(int argc, char **argv)
     p.profile = 1;   // per-phase times for the Timerdump file
     int jobid = 0;
     bool checksum = false, warmup = false, lazy = false, rdma = false, tile = true, higham = false;
-    bool higham_kappa = false, trail32 = false;
+    bool higham_kappa = false, trail32 = false, back_buffer = false;
     int device = 0;
     for (int i = 4; i < argc; ++i) {
         const char *a = argv[i];
@@ -117,6 +119,7 @@ int main(int argc, char **argv)
         else if (!std::strcmp(a, "-w") || !std::strcmp(a, "--warmup")) warmup = true;
         else if (!std::strcmp(a, "--checksum")) checksum = true;
         else if (!std::strcmp(a, "--trail32")) trail32 = true;
+        else if (!std::strcmp(a, "--back_buffer")) lazy = back_buffer = true;   // main.cpp:54
         else if (!std::strcmp(a, "--highammat")) higham = true;
         else if (!std::strcmp(a, "--highammat-kappa")) higham = higham_kappa = true;
         else if (!std::strcmp(a, "--fmt")) {
@@ -140,6 +143,7 @@ int main(int argc, char **argv)
     p.matrix = higham_kappa ? HPLAI_MATRIX_HIGHAM_KAPPA : (higham ? HPLAI_MATRIX_HIGHAM : HPLAI_MATRIX_HPLAI);
     // -d is the reference's double-decker layout (main.cpp:52-54): FP16 partial sums
     p.trail16 = lazy && !trail32 && b % 128 == 0 && p.fmt != HPLAI_FMT_F16_TILE && !higham ? 1 : 0;
+    if (back_buffer && p.trail16) p.trail16 = 2;
     p.cond = 1000.0;
 
     std::printf("jobid=%d\n", jobid);
@@ -149,7 +153,7 @@ int main(int argc, char **argv)
                 checksum ? "checksum" : "nocheck", higham ? "hmat" : "ddmat");
     std::printf("device=%d P=%d fmt=%s outer=%d lookahead=%d trailing=%s\n", device, nrow,
                 p.fmt == HPLAI_FMT_F16 ? "f16" : (p.fmt == HPLAI_FMT_F16_TILE ? "f16t" : "e4m3"),
-                p.outer, p.lookahead, p.trail16 ? "fp16" : "fp32");
+                p.outer, p.lookahead, p.trail16 == 2 ? "fp16+bb" : (p.trail16 ? "fp16" : "fp32"));
     std::printf("epoch_size = %d\n", b);   // b * lcm(1, 1) on a 1 x 1 grid (main.cpp:193)
     print_ctime("#BEGIN: ");
     std::fflush(stdout);

@@ -264,12 +264,13 @@ class Context:
         """matrix: "hplai" (the reference's LCG matrix), "higham" (--highammat as
         the reference generates it) or "higham_kappa" (Fasi and Higham's signs,
         cond(A, inf) = cond exactly); cond 0 = 1000.  trail16 (with lazy): the
-        partial sums of blocks not yet factored kept in FP16."""
+        partial sums of blocks not yet factored kept in FP16 (2: with the
+        compensated back buffer)."""
         p = Params(n=n, b=b, seed=seed, fmt=FMTS[fmt] if isinstance(fmt, str) else fmt, outer=outer,
                    lookahead=int(lookahead), lazy=1 if lazy else 0, maxit=maxit,
                    verbose=1 if verbose else 0, keep_factors=1 if keep_factors else 0,
                    profile=1 if profile else 0, matrix=MATRICES[matrix] if isinstance(matrix, str) else matrix,
-                   cond=cond, trail16=1 if trail16 else 0)
+                   cond=cond, trail16=int(trail16))
         r = Result()
         rc = lib().hplai_solve(self.handle, ctypes.byref(p), ctypes.byref(r))
         if rc not in (OK, ENOCONV):

@@ -229,7 +229,9 @@ typedef struct hplai_params {
     int      trail16;    /* 1 (with lazy = 1): the trailing updates keep the partial sums of
                             blocks not yet factored in an FP16 n x n array, scaled by
                             sqrt(n sqrt(n)) (the reference's -d double-decker lower deck,
-                            panel.hpp:45-83, panel_trf.hpp:343-345); b % 128 == 0           */
+                            panel.hpp:45-83, panel_trf.hpp:343-345); b % 128 == 0.
+                            2: also the compensated back buffer (--back_buffer,
+                            back_buffer.hpp:81-170), a second FP16 n x n array           */
 } hplai_params;
 
 #define HPLAI_MATRIX_HPLAI        0   /* the reference's LCG matrix with diagonal row sums */

@@ -442,22 +442,27 @@ def test_solve_packed_operands_match_oracle(ctx, oracle, fmt, outer, mode, tol, 
     assert rel < tol, rel
 
 
-@pytest.mark.parametrize("fmt,outer,lookahead", [("e4m3", 2, 1), ("f16", 2, 2), ("e4m3", 1, 0)])
-def test_solve_trail16(ctx, oracle, fmt, outer, lookahead):
+@pytest.mark.parametrize("fmt,b,outer,lookahead,mode", [("e4m3", 512, 2, 1, 1), ("f16", 512, 2, 2, 1),
+                                                         ("e4m3", 512, 1, 0, 1),
+                                                         # 64 outer steps: two back-buffer write-backs
+                                                         ("e4m3", 128, 1, 1, 2), ("f16", 128, 1, 1, 2)])
+def test_solve_trail16(ctx, oracle, fmt, b, outer, lookahead, mode):
     """16-bit trailing storage (lazy initialisation, partial sums of blocks not
-    yet factored in FP16 scaled by sqrt(n sqrt(n))).  The factors differ from the
-    oracle's FP32-partial-sum LU of the same format by the FP16 roundings of the
-    partial sums (at most one per outer step and element); measured ~1e-5
-    relative, bound 2e-4.  The solve still reaches the harness."""
-    n, b = 8192, 512
-    res = ctx.solve(n, b, fmt=fmt, outer=outer, lookahead=lookahead, lazy=True, trail16=True, keep_factors=True)
+    yet factored in FP16 scaled by sqrt(n sqrt(n))); mode 2 adds the back buffer
+    folded every 32 trailing updates.  The factors differ from the oracle's
+    FP32-partial-sum LU of the same format by the FP16 roundings of the partial
+    sums (at most one per outer step and element); measured ~2e-6 relative,
+    bound 2e-4 (a back buffer that dropped its deck would miss the partial sums,
+    ~1e-2 of the entries).  The solve still reaches the harness."""
+    n = 8192
+    res = ctx.solve(n, b, fmt=fmt, outer=outer, lookahead=lookahead, lazy=True, trail16=mode, keep_factors=True)
     lu = ctx.solution_factors(n)
     ctx.release()
     assert res["converged"] and res["harness"] < 16.0, res
     assert res["ir_steps"] <= 6
     ref, d = oracle.lu(oracle.LU_FP8 if fmt == "e4m3" else oracle.LU_PRECOMP, n, b, SEED)
     rel = np.max(np.abs(lu - ref)) / np.max(np.abs(ref))
-    print(fmt, outer, "trail16 factors vs oracle: %.3g" % rel)
+    print(fmt, b, outer, mode, "trail16 factors vs oracle: %.3g" % rel)
     assert rel < 2e-4, rel
 
 
@@ -513,7 +518,7 @@ def test_driver_output_and_timerdump(tmp_path):
     assert sec["TOTAL"] > 0 and sec["DIAG_LU"] > 0 and sec["GEMM_UPDATE"] > 0 and sec["IR_TRSV"] > 0
 
 
-@pytest.mark.parametrize("extra,trailing", [([], "fp16"), (["--trail32"], "fp32")])
+@pytest.mark.parametrize("extra,trailing", [([], "fp16"), (["--trail32"], "fp32"), (["--back_buffer"], "fp16+bb")])
 def test_driver_delayed_trailing_storage(tmp_path, extra, trailing):
     """driver.out -d with b % 128 == 0: the reference's double-decker layout,
     partial sums in FP16 (or FP32 with --trail32); both reach the harness."""
