@@ -753,6 +753,7 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
         // partial sums of not yet factored blocks in FP16, scaled by
         // scalea = sqrt(n sqrt(n)) (hpl_rand.hpp:154) and merged by the lazy
         // initialisation with downscale = 1 / scalea (panel_trf.hpp:343-345)
+        HPLAI_CHECK(p->trail16 == 1 || p->trail16 == 2, "hplai_solve: trail16 must be 0, 1 or 2");
         HPLAI_CHECK(p->lazy != 0, "hplai_solve: the FP16 trailing storage needs lazy initialisation");
         HPLAI_CHECK(p->matrix == HPLAI_MATRIX_HPLAI, "hplai_solve: the FP16 trailing storage takes the HPL-AI matrix");
         HPLAI_CHECK(fmt != HPLAI_FMT_F16_TILE && b % 128 == 0,

@@ -472,6 +472,8 @@ def test_solve_trail16_rejects_bad_options(ctx):
         ctx.solve(2048, 512, fmt="e4m3", lazy=False, trail16=True)
     with pytest.raises(hplai.HplaiError):
         ctx.solve(2048, 512, fmt="f16t", lazy=True, trail16=True)
+    with pytest.raises(hplai.HplaiError):
+        ctx.solve(2048, 512, fmt="e4m3", lazy=True, trail16=3)
 
 
 def test_solve_c1_lazy(ctx):
