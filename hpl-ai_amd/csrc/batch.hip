@@ -286,11 +286,18 @@ __device__ __noinline__ void batch_refine(const BatchSys &S, const float *LU, co
                              const double *B, double *X, double *R, double *xl, double *xs, double *red,
                              BatchOut &o, int maxit)
 {
-    const int n = S.n, lda = S.lda, tid = threadIdx.x;
+    // every right-hand side refines on its own (its own harness, steps and
+    // stop), but the residuals of all that still refine come from one pass
+    // over A64 per step: each element is read once and feeds up to 4 FMAs in
+    // the same order as a pass per right-hand side would
+    const int n = S.n, lda = S.lda, tid = threadIdx.x, nr = S.nrhs;
     double dm = 0.0;
     for (int i = tid; i < n; i += kBT) dm = fmax(dm, fabs(d[i]));
     const double norma = 2.0 * block_max(dm, red);          // hpl_infnorm, panel_norm.hpp:78-84
-    for (int rr = 0; rr < S.nrhs; ++rr) {
+    double normb[4] = {0.0, 0.0, 0.0, 0.0}, h[4] = {-1.0, -1.0, -1.0, -1.0};
+    int steps[4] = {0, 0, 0, 0};
+    bool done[4] = {false, false, false, false};
+    for (int rr = 0; rr < nr; ++rr) {
         const double *b = B + static_cast<int64_t>(rr) * n;
         double *x = X + static_cast<int64_t>(rr) * n;
         double bm = 0.0;
@@ -298,37 +305,61 @@ __device__ __noinline__ void batch_refine(const BatchSys &S, const float *LU, co
             x[i] = b[i] / d[i];                              // x0 = b ./ diag(A)
             bm = fmax(bm, fabs(b[i]));
         }
-        const double normb = block_max(bm, red);
-        double h = -1.0;
-        int steps = 0;
-        for (int it = 0; it < maxit; ++it) {
-            // r = b - A x in FP64 (x staged in LDS)
-            for (int i = tid; i < n; i += kBT) xl[i] = x[i];
-            __syncthreads();
-            double xm = 0.0, rm = 0.0;
-            for (int i = tid; i < n; i += kBT) {
-                double acc = b[i];
-                for (int j = 0; j < n; ++j) acc = fma(-A64[i + static_cast<int64_t>(j) * lda], xl[j], acc);
-                R[i] = acc;
-                rm = fmax(rm, fabs(acc));
-                xm = fmax(xm, fabs(xl[i]));
+        normb[rr] = block_max(bm, red);
+    }
+    for (int it = 0; it < maxit; ++it) {
+        // r = b - A x in FP64 for the right-hand sides still refining (x staged in LDS)
+        for (int rr = 0; rr < nr; ++rr)
+            if (!done[rr])
+                for (int i = tid; i < n; i += kBT) xl[rr * n + i] = X[static_cast<int64_t>(rr) * n + i];
+        __syncthreads();
+        double xm[4] = {0.0, 0.0, 0.0, 0.0}, rm[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = tid; i < n; i += kBT) {
+            double acc[4];
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) acc[rr] = rr < nr ? B[static_cast<int64_t>(rr) * n + i] : 0.0;
+            for (int j = 0; j < n; ++j) {
+                const double a = A64[i + static_cast<int64_t>(j) * lda];
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr)
+                    if (rr < nr) acc[rr] = fma(-a, xl[rr * n + j], acc[rr]);
             }
-            const double normr = block_max(rm, red);
-            const double normx = block_max(xm, red);
-            h = normr / (norma * normx + normb) * 1. / (static_cast<double>(n) * DBL_EPSILON / 2);
-            steps = it + 1;
-            if (h < 16.0) break;
-            batch_trsv<false, true>(n, lda, LU, R, xs);
-            batch_trsv<true, false>(n, lda, LU, R, xs);
-            for (int i = tid; i < n; i += kBT) x[i] += R[i];
-            __syncthreads();
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                if (rr < nr && !done[rr]) {
+                    R[static_cast<int64_t>(rr) * n + i] = acc[rr];
+                    rm[rr] = fmax(rm[rr], fabs(acc[rr]));
+                    xm[rr] = fmax(xm[rr], fabs(xl[rr * n + i]));
+                }
+            }
         }
-        if (tid == 0) {
-            o.harness[rr] = h;
-            o.steps[rr] = steps;
+        bool all = true;
+        for (int rr = 0; rr < nr; ++rr) {
+            if (done[rr]) continue;
+            const double normr = block_max(rm[rr], red);
+            const double normx = block_max(xm[rr], red);
+            h[rr] = normr / (norma * normx + normb[rr]) * 1. / (static_cast<double>(n) * DBL_EPSILON / 2);
+            steps[rr] = it + 1;
+            done[rr] = h[rr] < 16.0;
+            all = all && done[rr];
+        }
+        if (all) break;
+        for (int rr = 0; rr < nr; ++rr) {
+            if (done[rr]) continue;
+            double *r = R + static_cast<int64_t>(rr) * n;
+            batch_trsv<false, true>(n, lda, LU, r, xs);
+            batch_trsv<true, false>(n, lda, LU, r, xs);
+            double *x = X + static_cast<int64_t>(rr) * n;
+            for (int i = tid; i < n; i += kBT) x[i] += r[i];
         }
         __syncthreads();
     }
+    if (tid == 0)
+        for (int rr = 0; rr < nr; ++rr) {
+            o.harness[rr] = h[rr];
+            o.steps[rr] = steps[rr];
+        }
+    __syncthreads();
 }
 
 __global__ __launch_bounds__(kBT) void batch_kernel(const BatchSys *__restrict__ sys,
@@ -478,7 +509,7 @@ int hplai_batch_create(hplai_ctx ctx, int count, const int32_t *n, const int32_t
         S.x = dof;
         dof += up(static_cast<int64_t>(n[s]) * nrhs[s]);
         S.r = dof;
-        dof += up(n[s]);
+        dof += up(static_cast<int64_t>(n[s]) * nrhs[s]);
     }
     b->order.resize(count);
     for (int s = 0; s < count; ++s) b->order[s] = s;
