@@ -122,6 +122,44 @@ def cpu_baseline(n, b, threads):
                        "%.2f s, harness %.4f" % (n, b, t, r["final_harness"]))
 
 
+def cpu_reference_schur(threads, m=4096, k=512, cols=1024, reps=2):
+    """The reference's own CPU Schur kernel (shgemm_opt of kernels/avx2.cpp,
+    FP16 operands, FP32 C, C -= A B, compiled from the reference's sources
+    into oracle/_ref) on this host.  Like the reference's OpenMP Schur update
+    (schur_updator.hpp:425-447: each thread takes its own range of columns of
+    the trailing matrix) every thread updates its own `cols` columns of an m x
+    (threads * cols) C with K = k; ctypes releases the GIL during each call.
+    None when oracle/_ref was not built."""
+    import threading
+    import numpy as np
+    from oracle import oracle as O
+    R = O.ref()
+    if R is None:
+        return None
+    rng = np.random.default_rng(7)
+    a = (rng.random((k, m), dtype=np.float32) - 0.5).astype(np.float16)            # m x k, column-major
+    b = (rng.random((threads * cols, k), dtype=np.float32) - 0.5).astype(np.float16)   # k x n, column-major
+    c = np.zeros((threads * cols, m), dtype=np.float32)                             # m x n, column-major
+
+    def work(t):
+        for _ in range(reps):
+            R.ref_shgemm(m, cols, k, -1.0, a.ctypes.data, m, b[t * cols:].ctypes.data, k, 1.0,
+                         c[t * cols:].ctypes.data, m)
+
+    pool = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+    t0 = time.perf_counter()
+    for th in pool:
+        th.start()
+    for th in pool:
+        th.join()
+    t = time.perf_counter() - t0
+    flops = 2.0 * m * cols * k * threads * reps
+    return dict(value=flops / t * 1e-12, unit="TFLOP/s", cores=threads, kind="reference",
+                sample="reference kernels/avx2.cpp shgemm_opt, C(%d x %d) -= A(%d x %d) B, FP16 operands, "
+                       "FP32 C, %d threads x %d columns, %d passes: %.2f s" % (m, threads * cols, m, k,
+                                                                             threads, cols, reps, t))
+
+
 def bench_batch(args):
     """Batching form: 4096 seeded systems, sizes uniform in [32, 1024], 1-4
     right-hand sides; autotuned panel widths; the persistent launch replayed
@@ -307,6 +345,11 @@ def main():
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         cpu = cpu_baseline(args.cpu_n, args.cpu_b, threads)
+        ref_schur = cpu_reference_schur(threads)
+        if ref_schur is not None:
+            # the dominant GPU kernel against the reference's own CPU kernel for the same update
+            ref_schur["gpu_schur_tflops"] = roof["achieved"] if roof else None
+            cpu["reference_schur_kernel"] = ref_schur
 
     line = {
         "metric": "time-to-harness<16 & TFLOP/s at n=%d: FP8-scaled vs FP16 mixed LU, 1 GPU" % args.n,
