@@ -42,7 +42,8 @@ static void usage()
         "   --highammat-kappa Higham's matrix with cond(A, inf) = 1000 exactly.\n"
         "   --fmt <f>         Schur operands: f16 (default), f16t (per-tile 2^e), e4m3.\n"
         "   --outer <s>       Panels per trailing update (default 1).\n"
-        "   --lookahead <0|1> Factor panel k+1 on a second stream (default 1).\n"
+        "   --lookahead <n>   0: none; 1: factor panel k+1 on a second stream (default);\n"
+        "                     2: also update the next block row ahead of the trailing matrix.\n"
         "   --seed <s>        Generator seed (default 42).\n"
         "   --device <d>      HIP device (default 0).\n"
         "   --job-id <id>     Job id in the Timerdump file name (default 0).\n"
@@ -89,6 +90,11 @@ static void write_timerdump(bool warmup, int jobid, long long n, int b, const hp
 
 int main(int argc, char **argv)
 {
+    for (int i = 1; i < argc; ++i)
+        if (!std::strcmp(argv[i], "--help") || !std::strcmp(argv[i], "-h")) {
+            usage();
+            return 0;
+        }
     long long n = 120;
     int b = 3, nrow = 2;
     if (argc > 1) n = std::atoll(argv[1]);
@@ -135,7 +141,6 @@ int main(int argc, char **argv)
         else if (!std::strcmp(a, "--device")) device = std::atoi(next());
         else if (!std::strcmp(a, "--job-id")) jobid = std::atoi(next());
         else if (!std::strcmp(a, "--no-timerdump")) p.profile = 0;
-        else if (!std::strcmp(a, "--help") || !std::strcmp(a, "-h")) { usage(); return 0; }
         else std::printf("Ignoring unknown option: %s\n", a);
     }
     if (warmup) p.maxit = 1;
