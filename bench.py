@@ -100,7 +100,7 @@ def roofline(p, fmt, n):
             "algorithmic_bytes_per_launch": p["schur_large_bytes"] / nl,
             "all_schur_launches": {"launches": p["schur_launches"], "tflops": p["schur_tflops"],
                                    "ms": p["schur_ms"]},
-            "inner_k_b_launches": {"launches": p["schur_inner_launches"], "ms": p["schur_inner_ms"],
+            "in_block_launches": {"launches": p["schur_inner_launches"], "ms": p["schur_inner_ms"],
                                    "tflops": (p["schur_inner_flops"] / (p["schur_inner_ms"] * 1e-3) * 1e-12)
                                    if p["schur_inner_ms"] > 0 else None}}
 

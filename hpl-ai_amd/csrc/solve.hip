@@ -12,11 +12,11 @@
 //    step with K = B, which divides its HBM traffic by `outer`.  With outer = 1
 //    this is exactly the reference's right-looking schedule.
 // Per outer step (columns c0 .. c0+B):
-//   1. panel: for each inner block t: the rows below the outer block of
-//      block column t take the earlier inner blocks at once (left-looking,
-//      K = t b); getrf, L := A U^-1 (rows below), quantize L; U := L^-1 A for
-//      the rest of block row t inside the outer block, quantize, and update
-//      the square part of the outer block right of and below t (K = b);
+//   1. panel: for each inner block t: block column t (inside the outer block
+//      and below it) and then the rest of block row t inside the outer block
+//      take the earlier inner blocks at once (left-looking, K = t b); getrf,
+//      L := A U^-1 below the diagonal block, quantize L; U := L^-1 A for the
+//      rest of block row t inside the outer block, quantize;
 //   2. block row: for each t: row block t takes the earlier row blocks at
 //      once (left-looking, K = t b), then U12_t := L_tt^-1 A12_t, quantize;
 //   3. trailing: A22 -= Lq(:, 0..B) Uq(0..B, :) (K = B).
@@ -291,9 +291,10 @@ struct LU {
     const float *dinv(int64_t c) const { return B->inv + (c / b) * diag_inv_floats(b); }
 
     // trailing = true: an outer step's update of blocks that have not joined a
-    // panel yet, which goes to the FP16 trailing storage when it is on
+    // panel yet, which goes to the FP16 trailing storage when it is on;
+    // in_block = true: an update inside the outer panel (timed apart)
     int gemm(int buf, int t0, int nsl, int64_t row0, int64_t col0, int64_t m, int64_t nn,
-             hipStream_t s, bool trailing = false) const
+             hipStream_t s, bool trailing = false, bool in_block = false) const
     {
         if (m <= 0 || nn <= 0) return HPLAI_OK;
         if (trailing && trail16) {
@@ -310,7 +311,7 @@ struct LU {
         const double K = static_cast<double>(nsl) * b;
         timer->begin(s, 2.0 * static_cast<double>(m) * static_cast<double>(nn) * K,
                      static_cast<double>(m + nn) * K * es + 8.0 * static_cast<double>(m) * nn,
-                     schur_large_tiles(fmt, m, nn, b, num_cu), nsl == 1 && outer > 1);
+                     schur_large_tiles(fmt, m, nn, b, num_cu), in_block);
         const int rc = launch_schur(fmt, m, nn, nsl, kpad, lq(buf, row0, t0), ldq, el(buf, t0, row0),
                                     ntiles, uq(buf, col0, t0), ldq, eu(buf, t0, col0), ntiles, b,
                                     a(row0, col0), lda, num_cu, s, packed);
@@ -344,8 +345,8 @@ struct LU {
 
     // Step 1 for the outer block at c0 with sw inner blocks.  Two streams:
     // s carries the chain from one diagonal block to the next (getrf, the
-    // inverses, the TRSMs and quantization of the rows and columns inside the
-    // outer block, the in-block update); the rows below the outer block (their
+    // inverses, the left-looking updates, TRSMs and quantization of the rows
+    // and columns inside the outer block); the rows below the outer block (their
     // left-looking update, lazy initialisation, TRSM and quantization) go to
     // the bulk stream, which waits for the chain only where it needs its
     // results.  s waits for the bulk stream at the end.
@@ -378,7 +379,9 @@ struct LU {
                 }
                 if (rc == HPLAI_OK) rc = lazy_init(rb0, c, mb, b, q);
             }
-            // chain: the diagonal block and the rows of the block column inside the outer block
+            // chain: the diagonal block and the rows of the block column inside
+            // the outer block take the t earlier inner blocks at once (K = t b)
+            if (rc == HPLAI_OK && t > 0) rc = gemm(buf, 0, t, c, c, rb0 - c, b, s, false, true);
             if (rc == HPLAI_OK) rc = lazy_init(c, c, rb0 - c, b, s);
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_DIAG_LU, 2.0 / 3.0 * bd * bd * bd, s, [&] {
@@ -419,7 +422,9 @@ struct LU {
                 });
             const int64_t ncols = rb0 - (c + b);
             if (rc) break;
-            rc = lazy_init(c, c + b, b, ncols, s);   // block row inside the panel
+            // block row t inside the outer block, likewise from the left (K = t b)
+            if (t > 0) rc = gemm(buf, 0, t, c, c + b, b, ncols, s, false, true);
+            if (rc == HPLAI_OK) rc = lazy_init(c, c + b, b, ncols, s);
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_TRSM_R, static_cast<double>(ncols) * bd * bd, s, [&] {
                     return launch_trsm_u(b, ncols, a(c, c), lda, a(c, c + b), lda, amu(buf, t, c + b), b,
@@ -432,9 +437,6 @@ struct LU {
                 });
             if (rc) break;
             if (mb > 0) HPLAI_HIP(hipEventRecord(ev_u, s));
-            // right-looking inside the outer block only (rows and columns
-            // c + b .. c0 + Bw, K = b)
-            rc = gemm(buf, t, 1, c + b, c + b, ncols, ncols, s);
         }
         if (mb > 0) {
             HPLAI_HIP(hipEventRecord(ev_join, q));

@@ -290,7 +290,8 @@ typedef struct hplai_result {
     double  schur_large_flops;
     double  schur_large_ms;
     double  schur_large_bytes;
-    /* the launches with K = b (updates inside the outer panel and its block row) */
+    /* the updates inside the outer panel (block column and row block t take
+       the t earlier inner blocks, K = t b, on the panel's stream) */
     int64_t schur_inner_launches;
     double  schur_inner_flops;
     double  schur_inner_ms;
