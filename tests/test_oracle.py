@@ -147,6 +147,27 @@ def test_e4m3_matches_bruteforce(oracle):
     assert np.array_equal(got, want)
 
 
+def test_e4m3_matches_torch_float8(oracle):
+    """The oracle's FP32 -> e4m3 rounding agrees bit for bit with torch's
+    float8_e4m3fn cast (round to nearest even) for every value up to the
+    largest finite 448, every halfway point between adjacent codes included
+    (beyond 448 torch gives NaN where the quantizer saturates; tile scaling
+    keeps the quantizer's inputs at or below 448)."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(5)
+    x = np.concatenate([
+        np.clip(rng.standard_normal(4000) * 60, -448, 448),
+        rng.standard_normal(2000) * 1e-2,
+        rng.standard_normal(2000) * 3e-3,
+        [0.0, -0.0, 448.0, -448.0, 2 ** -9, 2 ** -10, 3 * 2 ** -11, 2 ** -6, 1.0625]]).astype(np.float32)
+    grid = np.array(sorted(set(abs(float(oracle.e4m3_to_f32(q))) for q in range(127))))
+    mids = ((grid[:-1] + grid[1:]) / 2).astype(np.float32)
+    x = np.concatenate([x, mids, -mids])
+    got = oracle.f32_to_e4m3_bits(x)
+    want = torch.from_numpy(x).to(torch.float8_e4m3fn).view(torch.uint8).numpy()
+    assert np.array_equal(got, want)
+
+
 def test_e4m3_decode_table(oracle):
     assert oracle.e4m3_to_f32(0x7E) == 448.0
     assert oracle.e4m3_to_f32(0x01) == 2.0 ** -9
