@@ -78,24 +78,40 @@ __device__ __noinline__ void batch_generate(const BatchSys &S, float *A32, doubl
     // thread t takes e = t + 256 q, advancing 256 states per step
     const LcgCoeff step = kLcg.p[8];      // (a, c)^256
     uint64_t s = lcg_jump(S.x1, static_cast<uint64_t>(tid));
-    const int64_t nn = static_cast<int64_t>(n) * n;
+    // (i, j) of element e, advanced with e (one division at the start)
+    int64_t j = tid / n, i = tid - j * n;
     for (int64_t e = tid; e < total; e += kBT) {
         const double v = lcg_value(s);
-        if (e < nn) {
-            const int64_t j = e / n, i = e - j * n;
+        if (j < n) {
             A64[i + j * lda] = v;
             A32[i + j * lda] = static_cast<float>(v);
         } else {
-            B[e - nn] = v;
+            B[(j - n) * n + i] = v;
         }
         s = step.a * s + step.c;
+        i += kBT;
+        while (i >= n) {
+            i -= n;
+            ++j;
+        }
     }
     __syncthreads();
-    // calc_diag (hpl_rand.hpp:108-123): FP64, j ascending, j != i
+    // calc_diag (hpl_rand.hpp:108-123): FP64, j ascending, j != i; the loads
+    // of 16 columns are issued together ahead of their adds
     for (int i = tid; i < n; i += kBT) {
+        const double *arow = A64 + i;
         double sum = 0.0;
-        for (int j = 0; j < n; ++j)
-            if (j != i) sum += fabs(A64[i + static_cast<int64_t>(j) * lda]);
+        int j = 0;
+        for (; j + 16 <= n; j += 16) {
+            double a[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) a[q] = arow[static_cast<int64_t>(j + q) * lda];
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                if (j + q != i) sum += fabs(a[q]);
+        }
+        for (; j < n; ++j)
+            if (j != i) sum += fabs(arow[static_cast<int64_t>(j) * lda]);
         d[i] = sum;
         A64[i + static_cast<int64_t>(i) * lda] = sum;
         A32[i + static_cast<int64_t>(i) * lda] = static_cast<float>(sum);
@@ -132,9 +148,26 @@ __device__ __noinline__ int batch_lu(int n, int lda, float *A, uint8_t *lds)
     for (int k = 0; k < n; k += NB) {
         const int w = n - k < NB ? n - k : NB;
         const int mp = n - k;
-        // (a) panel into LDS
-        for (int c = 0; c < w; ++c)
-            for (int r = tid; r < mp; r += kBT) P[c * mp + r] = A[(k + r) + static_cast<int64_t>(k + c) * lda];
+        // (a) panel into LDS, 4 columns at a time: the (up to 16) loads of a
+        // thread are issued together ahead of their LDS stores
+        constexpr int RPT = (kBatchMaxN + kBT - 1) / kBT;   // panel rows per thread (mp <= n)
+        for (int c0 = 0; c0 < w; c0 += 4) {
+            float v[4][RPT];
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    const int c = c0 + cc, r = tid + q * kBT;
+                    v[cc][q] = (c < w && r < mp) ? A[(k + r) + static_cast<int64_t>(k + c) * lda] : 0.0f;
+                }
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    const int c = c0 + cc, r = tid + q * kBT;
+                    if (c < w && r < mp) P[c * mp + r] = v[cc][q];
+                }
+        }
         __syncthreads();
         // (a') unblocked LU of the tall panel (getrf_nopiv_small order per column)
         for (int c = 0; c < w; ++c) {
@@ -318,8 +351,21 @@ __device__ __noinline__ void batch_refine(const BatchSys &S, const float *LU, co
             double acc[4];
 #pragma unroll
             for (int rr = 0; rr < 4; ++rr) acc[rr] = rr < nr ? B[static_cast<int64_t>(rr) * n + i] : 0.0;
-            for (int j = 0; j < n; ++j) {
-                const double a = A64[i + static_cast<int64_t>(j) * lda];
+            // j ascending; the loads of 16 columns are issued together
+            const double *arow = A64 + i;
+            int j = 0;
+            for (; j + 16 <= n; j += 16) {
+                double a[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) a[q] = arow[static_cast<int64_t>(j + q) * lda];
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr)
+                        if (rr < nr) acc[rr] = fma(-a[q], xl[rr * n + j + q], acc[rr]);
+            }
+            for (; j < n; ++j) {
+                const double a = arow[static_cast<int64_t>(j) * lda];
 #pragma unroll
                 for (int rr = 0; rr < 4; ++rr)
                     if (rr < nr) acc[rr] = fma(-a, xl[rr * n + j], acc[rr]);
