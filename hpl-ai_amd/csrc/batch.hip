@@ -296,22 +296,127 @@ __device__ __noinline__ int batch_lu(int n, int lda, float *A, uint8_t *lds)
     return info;
 }
 
-// each size class in its own function, so each gets its own register budget
+// The triangular solve of trsv_block (trsv_block.hpp) for the nr <= 4
+// right-hand sides of a system at once (columns x + rr bs): every strip of
+// the triangle is loaded once for all of them, and each right-hand side
+// takes exactly trsv_block's operations in its order.  xs: 4 x 32 doubles
+// of LDS.
 template <int RPT, bool UPPER, bool UNIT>
-__device__ __noinline__ void batch_trsv_rpt(int n, int lda, const float *A, double *x, double *xs)
+__device__ __noinline__ void batch_trsv_multi(int bs, int lda, const float *__restrict__ T,
+                                              double *__restrict__ x, int nr, double *xs)
 {
-    trsv_block<RPT, UPPER, UNIT, false>(n, A, lda, x, xs);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double xr[4][RPT];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int r = tid + 256 * k;
+            xr[rr][k] = (rr < nr && r < bs) ? x[static_cast<int64_t>(rr) * bs + r] : 0.0;
+        }
+    const int nsb = (bs + 31) / 32;
+    constexpr bool PF = RPT <= 2;
+    float cur[RPT][32], nxt[PF ? RPT : 1][32];
+    trsv_load_strip<RPT, false>(T, lda, bs, (UPPER ? nsb - 1 : 0) * 32, tid, cur);
+    for (int step = 0; step < nsb; ++step) {
+        const int q = UPPER ? nsb - 1 - step : step;
+        const int sb = q * 32;
+        const int w = bs - sb < 32 ? bs - sb : 32;
+        if constexpr (PF)
+            if (step + 1 < nsb) trsv_load_strip<RPT, false>(T, lda, bs, (UPPER ? q - 1 : q + 1) * 32, tid, nxt);
+        const int kq = sb / 256, owner = (sb % 256) / 64, half = ((sb % 256) / 32) & 1;
+        if (wave == owner) {
+            const int i = lane - 32 * half;
+            const bool mine = i >= 0 && i < w;
+            double v[4] = {0.0, 0.0, 0.0, 0.0}, rinv = 1.0;
+            float rowv[32];
+#pragma unroll
+            for (int k = 0; k < RPT; ++k)
+                if (k == kq) {
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) v[rr] = xr[rr][k];
+#pragma unroll
+                    for (int c = 0; c < 32; ++c) rowv[c] = cur[k][c];
+                }
+            if (!UNIT) {
+                float dg = 1.0f;
+#pragma unroll
+                for (int c = 0; c < 32; ++c)
+                    if (c == i) dg = rowv[c];
+                rinv = 1.0 / static_cast<double>(dg);
+            }
+#pragma unroll
+            for (int cc = 0; cc < 32; ++cc) {
+                const int c = UPPER ? 31 - cc : cc;
+                if (c < w) {
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) {
+                        if (rr < nr) {
+                            if (!UNIT && i == c) v[rr] *= rinv;
+                            const double xc = readlane_f64(v[rr], 32 * half + c);
+                            if (mine && (UPPER ? i < c : i > c))
+                                v[rr] = fma(-static_cast<double>(rowv[c]), xc, v[rr]);
+                        }
+                    }
+                }
+            }
+            if (mine)
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) xs[rr * 32 + i] = v[rr];
+#pragma unroll
+            for (int k = 0; k < RPT; ++k)
+                if (k == kq && mine)
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) xr[rr][k] = v[rr];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int r = tid + 256 * k;
+            const bool further = UPPER ? (r < sb) : (r >= sb + w && r < bs);
+            if (further) {
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    if (rr < nr) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int c = 0; c < 32; ++c)
+                            if (c < w) acc = fma(static_cast<double>(cur[k][c]), xs[rr * 32 + c], acc);
+                        xr[rr][k] -= acc;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (step + 1 < nsb) {
+            if constexpr (PF) {
+#pragma unroll
+                for (int k = 0; k < RPT; ++k)
+#pragma unroll
+                    for (int c = 0; c < 32; ++c) cur[k][c] = nxt[k][c];
+            } else {
+                trsv_load_strip<RPT, false>(T, lda, bs, (UPPER ? q - 1 : q + 1) * 32, tid, cur);
+            }
+        }
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int r = tid + 256 * k;
+            if (rr < nr && r < bs) x[static_cast<int64_t>(rr) * bs + r] = xr[rr][k];
+        }
 }
 
 template <bool UPPER, bool UNIT>
-__device__ void batch_trsv(int n, int lda, const float *A, double *x, double *xs)
+__device__ void batch_trsv_all(int n, int lda, const float *A, double *x, int nr, double *xs)
 {
     if (n <= 256)
-        batch_trsv_rpt<1, UPPER, UNIT>(n, lda, A, x, xs);
+        batch_trsv_multi<1, UPPER, UNIT>(n, lda, A, x, nr, xs);
     else if (n <= 512)
-        batch_trsv_rpt<2, UPPER, UNIT>(n, lda, A, x, xs);
+        batch_trsv_multi<2, UPPER, UNIT>(n, lda, A, x, nr, xs);
     else
-        batch_trsv_rpt<4, UPPER, UNIT>(n, lda, A, x, xs);
+        batch_trsv_multi<4, UPPER, UNIT>(n, lda, A, x, nr, xs);
 }
 
 // ---- FP64 refinement of every right-hand side (iterative_refinement.hpp:18-57) ----
@@ -390,11 +495,13 @@ __device__ __noinline__ void batch_refine(const BatchSys &S, const float *LU, co
             all = all && done[rr];
         }
         if (all) break;
+        // corrections of all right-hand sides in one pass over each triangle
+        // (the ones already done are solved too and not used)
+        batch_trsv_all<false, true>(n, lda, LU, R, nr, xs);
+        batch_trsv_all<true, false>(n, lda, LU, R, nr, xs);
         for (int rr = 0; rr < nr; ++rr) {
             if (done[rr]) continue;
-            double *r = R + static_cast<int64_t>(rr) * n;
-            batch_trsv<false, true>(n, lda, LU, r, xs);
-            batch_trsv<true, false>(n, lda, LU, r, xs);
+            const double *r = R + static_cast<int64_t>(rr) * n;
             double *x = X + static_cast<int64_t>(rr) * n;
             for (int i = tid; i < n; i += kBT) x[i] += r[i];
         }
@@ -414,7 +521,7 @@ __global__ __launch_bounds__(kBT) void batch_kernel(const BatchSys *__restrict__
                                                     BatchOut *__restrict__ out, int32_t *counter, int maxit)
 {
     extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
-    __shared__ double xs[32];
+    __shared__ double xs[4 * 32];
     __shared__ double red[4];
     __shared__ int32_t next;
     for (;;) {

@@ -1,7 +1,8 @@
 // trsv_block.hpp — one workgroup solves a triangular system of order <= 1024
 // with FP32 factor entries and an FP64 vector (the diagonal-block step of
 // panel_trsvL / panel_trsvU, panel_trsv.hpp:310-758).  Shared by the
-// blocked triangular solves and the batched solver.
+// blocked triangular solves; the batched solver runs the same steps for
+// several right-hand sides at once (batch.hip).
 //
 // Thread t owns rows t + 256 k and keeps their x in registers.  The triangle
 // is walked in 32-column strips; every thread holds its rows' 32 entries of
