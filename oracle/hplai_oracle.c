@@ -1,7 +1,7 @@
 /*
  * hplai_oracle.c — CPU restatement of HPL-AI's mixed-precision solve.
  *
- * TEST INFRASTRUCTURE ONLY (the checker, never the product): loaded by
+ * TEST INFRASTRUCTURE ONLY (the oracle, never the product): loaded by
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
  *
  * Each function cites the reference file:line (RIKEN-RCCS/hpl-ai) whose

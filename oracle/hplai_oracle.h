@@ -1,8 +1,8 @@
 /*
  * hplai_oracle.h — CPU restatement of HPL-AI's mixed-precision solve path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
- * implementation in hpl-ai_amd/: only tests/, __graft_entry__.smoke() and
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle the HIP
+ * implementation in hpl-ai_amd/ is tested against: only tests/, __graft_entry__.smoke() and
  * bench.py's cpu_baseline leg may load it.  The product path never calls it.
  *
  * Every function restates the reference algorithm (RIKEN-RCCS/hpl-ai) and

@@ -1,7 +1,7 @@
 """ctypes wrapper over the CPU oracle (oracle/libhplai_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product (hpl-ai_amd/) never
+cpu_baseline leg of bench.py, to check results against.  The product (hpl-ai_amd/) never
 imports it.
 
 Also exposes the reference-source shim (oracle/_ref/libhplai_ref.so) when it
