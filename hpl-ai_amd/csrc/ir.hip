@@ -188,7 +188,16 @@ __global__ __launch_bounds__(256) void trsv_gemv_t_kernel(int64_t m, int w, cons
         if (r >= m) break;
         const float *pr = P + r * lda;
         double acc = 0.0;
-        for (int c = lane; c < w; c += 64) acc = fma(static_cast<double>(pr[c]), xs[c], acc);
+        // columns lane, lane + 64, ... in that order; 8 loads issued together
+        int c = lane;
+        for (; c + 64 * 7 < w; c += 64 * 8) {
+            float v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = pr[c + 64 * q];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc = fma(static_cast<double>(v[q]), xs[c + 64 * q], acc);
+        }
+        for (; c < w; c += 64) acc = fma(static_cast<double>(pr[c]), xs[c], acc);
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
         if (lane == 0) y[r] -= acc;
