@@ -42,7 +42,21 @@ __global__ __launch_bounds__(kGemvRows) void otf_gemv_partial_kernel(
     const int64_t len = j1 - j0;
     const int64_t idiag = i - j0;   // local column of the diagonal, if in this chunk
     int64_t t = 0;
-    for (; t + 1 < len; t += 2) {
+    // pairs (t, t + 1) with no diagonal element up to te; the one pair that
+    // holds the diagonal (when it is in this chunk) is masked on its own
+    auto pairs = [&](int64_t te) {
+        for (; t + 1 < te; t += 2) {
+            const double a0 = static_cast<double>(static_cast<int64_t>(s0));
+            const double a1 = static_cast<double>(static_cast<int64_t>(s1));
+            acc0 = fma(a0, xs[t], acc0);
+            acc1 = fma(a1, xs[t + 1], acc1);
+            s0 = jn2.a * s0 + jn2.c;
+            s1 = jn2.a * s1 + jn2.c;
+        }
+    };
+    const int64_t pd = (idiag >= 0 && idiag < len) ? (idiag & ~static_cast<int64_t>(1)) : len;
+    pairs(pd);
+    if (t == pd && t + 1 < len) {
         double a0 = static_cast<double>(static_cast<int64_t>(s0));
         double a1 = static_cast<double>(static_cast<int64_t>(s1));
         a0 = (t == idiag) ? 0.0 : a0;
@@ -51,7 +65,9 @@ __global__ __launch_bounds__(kGemvRows) void otf_gemv_partial_kernel(
         acc1 = fma(a1, xs[t + 1], acc1);
         s0 = jn2.a * s0 + jn2.c;
         s1 = jn2.a * s1 + jn2.c;
+        t += 2;
     }
+    pairs(len);
     if (t < len) {
         double a0 = static_cast<double>(static_cast<int64_t>(s0));
         a0 = (t == idiag) ? 0.0 : a0;
