@@ -5,6 +5,8 @@
                                     --kernel NAME_SUBSTRING --copy-bytes BYTES
     python -m hplai.profsum launches KERNEL_TRACE_CSV --kernel NAME_SUBSTRING
                                     [--first N]
+    python -m hplai.profsum hbm --fetch FETCH_CSV --write WRITE_CSV
+                                --copy-bytes BYTES --kernel SUBSTRING [--kernel ...]
 
 `stats` reads the `*_kernel_stats.csv` written by `rocprofv3 --kernel-trace
 --stats --output-format csv` and prints one line per kernel (calls, total and
@@ -24,6 +26,12 @@ calibrated: the factor is what FETCH_SIZE (WRITE_SIZE) reads for that copy
 (summed over its dispatches) divided by BYTES.  The HBM bytes per launch of the kernel whose name contains
 NAME_SUBSTRING are then FETCH/factor_f + WRITE/factor_w, averaged over its
 launches.
+
+`hbm` applies the same calibration to several kernels and divides each
+one's HBM bytes by its summed dispatch durations from the FETCH_SIZE run
+(counter collection runs the dispatches one at a time, so these are
+standalone durations): launches, HBM bytes per launch, mean duration and
+the achieved HBM rate in GB/s.
 """
 import argparse
 import csv
@@ -100,6 +108,36 @@ def traffic(fetch, write, kernel, copy_bytes, copy_kernel="MulFunctor"):
     print("hbm_bytes_per_launch %.6g" % (fetch_per + write_per))
 
 
+def _durations(path, counter):
+    """{dispatch_id: duration in ns} from the rows of one counter."""
+    out = {}
+    for r in _rows(path):
+        if r.get("Counter_Name") == counter:
+            out[int(r["Dispatch_Id"])] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    return out
+
+
+def hbm(fetch, write, kernels, copy_bytes, copy_kernel="MulFunctor"):
+    pf = _per_dispatch(fetch, "FETCH_SIZE")
+    pw = _per_dispatch(write, "WRITE_SIZE")
+    ff = _copy_factor(pf, copy_bytes, copy_kernel)
+    fw = _copy_factor(pw, copy_bytes, copy_kernel)
+    dur = _durations(fetch, "FETCH_SIZE")
+    print("fetch_factor %.4f write_factor %.4f (copy of %d bytes)" % (ff, fw, copy_bytes))
+    print("%-44s %8s %14s %12s %10s" % ("kernel", "launches", "hbm_B/launch", "mean_us", "GB/s"))
+    for k in kernels:
+        ids = [d for d, (name, _) in pf.items() if k in name]
+        wv = [v for name, v in pw.values() if k in name]
+        if not ids or not wv:
+            raise SystemExit("kernel %r not found" % k)
+        fbytes = sum(pf[d][1] for d in ids) / ff
+        wbytes = sum(wv) / fw * len(ids) / len(wv)   # the write run's launches, scaled to the same count
+        ns = sum(dur[d] for d in ids)
+        per = (fbytes + wbytes) / len(ids)
+        print("%-44s %8d %14.6g %12.2f %10.1f" % (k[:44], len(ids), per, ns / len(ids) * 1e-3,
+                                                  (fbytes + wbytes) / ns if ns > 0 else 0.0))
+
+
 def launches(path, kernel, first):
     rows = [r for r in _rows(path) if kernel in r["Kernel_Name"]]
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
@@ -131,11 +169,19 @@ def main(argv=None):
     g.add_argument("csv")
     g.add_argument("--kernel", required=True)
     g.add_argument("--first", type=int, default=0)
+    h = sub.add_parser("hbm")
+    h.add_argument("--fetch", required=True)
+    h.add_argument("--write", required=True)
+    h.add_argument("--kernel", action="append", required=True)
+    h.add_argument("--copy-bytes", type=int, required=True)
+    h.add_argument("--copy-kernel", default="MulFunctor")
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         stats(a.csv, a.top)
     elif a.cmd == "launches":
         launches(a.csv, a.kernel, a.first)
+    elif a.cmd == "hbm":
+        hbm(a.fetch, a.write, a.kernel, a.copy_bytes, a.copy_kernel)
     else:
         traffic(a.fetch, a.write, a.kernel, a.copy_bytes, a.copy_kernel)
 

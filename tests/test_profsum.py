@@ -41,3 +41,34 @@ def test_traffic_sums_split_calibration_copy(tmp_path, capsys):
     assert float(out["write_factor"]) == 1.0
     # mean fetch reading 4 GiB / 0.5 + mean write reading 1 GiB / 1.0
     assert abs(float(out["hbm_bytes_per_launch"]) / (9 * gib) - 1.0) < 1e-5
+
+
+def test_hbm_rate_uses_calibrated_bytes_and_fetch_run_durations(tmp_path, capsys):
+    gib = 1 << 30
+    mul = "vectorized_elementwise_kernel<4, MulFunctor<float>>"
+    fields = FIELDS + ["Start_Timestamp", "End_Timestamp"]
+
+    def write(path, counter, rows):
+        with open(path, "w", newline="") as f:
+            w = csv.DictWriter(f, fieldnames=fields)
+            w.writeheader()
+            for d, name, kib, t0, t1 in rows:
+                w.writerow({"Dispatch_Id": d, "Kernel_Name": name, "Counter_Name": counter,
+                            "Counter_Value": kib, "Start_Timestamp": t0, "End_Timestamp": t1})
+
+    # calibration: 4 GiB read (FETCH reads half) and written (WRITE reads all)
+    fetch = [(1, mul, 2 * gib // 1024, 0, 10), (2, "quant_l16_kernel<2, true>", gib // 1024, 100, 100 + 1000),
+             (3, "quant_l16_kernel<2, true>", gib // 1024, 2000, 2000 + 3000)]
+    wr = [(1, mul, 4 * gib // 1024, 0, 10), (2, "quant_l16_kernel<2, true>", gib // 2048, 0, 1),
+          (3, "quant_l16_kernel<2, true>", gib // 2048, 0, 1)]
+    pf, pw = tmp_path / "f.csv", tmp_path / "w.csv"
+    write(pf, "FETCH_SIZE", fetch)
+    write(pw, "WRITE_SIZE", wr)
+    profsum.hbm(str(pf), str(pw), ["quant_l16_kernel<2, true>"], 4 * gib)
+    line = capsys.readouterr().out.strip().splitlines()[-1].split()
+    launches, per, mean_us, rate = int(line[-4]), float(line[-3]), float(line[-2]), float(line[-1])
+    # per launch: 1 GiB fetched reading / 0.5 + 0.5 GiB written = 2.5 GiB; 4000 ns over two launches
+    assert launches == 2
+    assert abs(per / (2.5 * gib) - 1) < 1e-5
+    assert abs(mean_us - 2.0) < 1e-9
+    assert abs(rate / (2 * 2.5 * gib / 4000) - 1) < 1e-5
