@@ -114,6 +114,18 @@ constexpr int kE4m3Er = 8;
 constexpr uint32_t kF16MrFrac = 0x7fe000u;   // 65504 = (2 - 2^-10) * 2^15
 constexpr int kF16Er = 15;
 
+// e4m3 exponents become the E8M0 block scales 127 - e of the scaled MFMA, so
+// they stay in [-127, 127]: a tile whose largest magnitude is below about
+// 2^-119 (e.g. subnormal) takes e = 127 and quantizes to (near) zero instead
+// of producing the NaN scale byte 0xFF.
+__host__ __device__ inline int clamp_e4m3_exponent(int e) { return e > 127 ? 127 : (e < -127 ? -127 : e); }
+__host__ __device__ inline int e4m3_exponent(float amax)
+{
+    return clamp_e4m3_exponent(tile_exponent(amax, kE4m3MrFrac, kE4m3Er));
+}
+// E8M0 scale byte of exponent e (caller-supplied exponents are clamped too)
+__host__ __device__ inline int e8m0_scale(int e) { return 127 - clamp_e4m3_exponent(e); }
+
 __device__ __forceinline__ float ldexp_fast(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
 
 // Panel-packed layout of the solver's quantized operand panels: rows in

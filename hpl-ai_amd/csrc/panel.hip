@@ -569,8 +569,7 @@ __device__ __forceinline__ int fmt_exponent(const float *amax, int64_t tile)
 {
     if (FMT == HPLAI_FMT_F16) return 0;
     const float a = amax[tile];
-    return FMT == HPLAI_FMT_E4M3 ? tile_exponent(a, kE4m3MrFrac, kE4m3Er)
-                                 : tile_exponent(a, kF16MrFrac, kF16Er);
+    return FMT == HPLAI_FMT_E4M3 ? e4m3_exponent(a) : tile_exponent(a, kF16MrFrac, kF16Er);
 }
 
 template <int FMT, bool PACKED>

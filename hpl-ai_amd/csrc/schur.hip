@@ -301,8 +301,8 @@ __global__ __launch_bounds__(kThreads) void schur_kernel(
             const int s = ks / stages_per_slice;
             const int el = eL[s * ldel + row_tile], eu = eU[s * ldeu + col_tile];
             if (FMT == HPLAI_FMT_E4M3) {
-                scale_a = 127 - el;
-                scale_b = 127 - eu;
+                scale_a = e8m0_scale(el);
+                scale_b = e8m0_scale(eu);
             } else if (FMT == HPLAI_FMT_F16_TILE) {
                 const int e_new = el + eu;
                 if (ks == 0) {
@@ -561,8 +561,8 @@ __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
         }
     };
     auto take_exps = [&](Ops &o) {
-        o.sa = 127 - pe_l;
-        o.sb = 127 - pe_u;
+        o.sa = e8m0_scale(pe_l);
+        o.sb = e8m0_scale(pe_u);
     };
     // MFMAs of accumulator rows i0, i0 + 1
     auto mma = [&](const Ops &o, int i0) {

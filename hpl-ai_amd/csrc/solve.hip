@@ -717,6 +717,21 @@ int hplai_solution_factors(hplai_ctx ctx, float *lu_host, int64_t ldh)
     return HPLAI_OK;
 }
 
+int hplai_solution_block(hplai_ctx ctx, int64_t i0, int64_t j0, int64_t m, int64_t k, float *host,
+                         int64_t ldh)
+{
+    HPLAI_CHECK(ctx != nullptr && ctx->A != nullptr && host != nullptr,
+                "solution_block: no kept factors (run hplai_solve with keep_factors)");
+    HPLAI_CHECK(i0 >= 0 && j0 >= 0 && m >= 0 && k >= 0 && i0 + m <= ctx->n && j0 + k <= ctx->n && ldh >= m,
+                "solution_block: block (%lld, %lld) of %lld x %lld outside the %lld x %lld factors or ldh < m",
+                (long long)i0, (long long)j0, (long long)m, (long long)k, (long long)ctx->n, (long long)ctx->n);
+    if (m == 0 || k == 0) return HPLAI_OK;
+    HPLAI_HIP(hipSetDevice(ctx->device));
+    HPLAI_HIP(hipMemcpy2D(host, sizeof(float) * ldh, ctx->A + j0 * ctx->lda + i0, sizeof(float) * ctx->lda,
+                          sizeof(float) * m, k, hipMemcpyDeviceToHost));
+    return HPLAI_OK;
+}
+
 int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
 {
     HPLAI_CHECK(ctx != nullptr && p != nullptr && res != nullptr, "hplai_solve: NULL argument");
@@ -800,117 +815,125 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
         return rc;
     }
 
-    // ---- generation (main.cpp:256-275), outside the timed region ----
-    HPLAI_HIP(hipDeviceSynchronize());
-    double t0 = now_s();
-    if (rc == HPLAI_OK)
-        rc = lu.higham ? launch_higham_diag(lu.h_alpha, lu.h_beta, 0, n, B.d, s)   // pdiaggen(HMGen)
-                       : launch_gen_diag(n, p->seed, 0, n, B.d, s);
-    if (lu.lazy) {
-        // pmatgen0 (matgen.hpp:80-105): partial sums start at zero
-        HPLAI_HIP(hipMemsetAsync(B.A, 0, sizeof(float) * lu.lda * n, s));
-        if (lu.trail16) HPLAI_HIP(hipMemsetAsync(B.P, 0, sizeof(uint16_t) * n * n, s));
-        if (lu.bback) HPLAI_HIP(hipMemsetAsync(B.Q, 0, sizeof(uint16_t) * n * n, s));
-    } else if (rc == HPLAI_OK) {
-        rc = lu.higham ? launch_gen_higham(lu.h_alpha, lu.h_beta, 0, 0, n, n, B.A, lu.lda, 0, s)
-                       : launch_gen_matrix(n, p->seed, 0, 0, n, n, B.d, B.A, lu.lda, 0, s);
-    }
-    if (rc == HPLAI_OK) rc = launch_gen_rhs(n, p->seed, 0, n, B.rhs, s);
-    HPLAI_HIP(hipDeviceSynchronize());
-    double t1 = now_s();
-    res->time_gen = t1 - t0;
-
-    // ---- LU ----
-    if (rc == HPLAI_OK) rc = lu.run(p->lookahead);
-    HPLAI_HIP(hipDeviceSynchronize());
-    epochs.flush(true);
-    double t2 = now_s();
-    res->time_lu = t2 - t1;
-    if (rc == HPLAI_OK) rc = timer.rc;
-    timer.collect(res);
-    if (rc == HPLAI_OK) rc = check_pivots(lu, B, "hplai_solve");
-
-    // ---- iterative refinement (iterative_refinement.hpp:18-57) ----
-    int steps = 0;
+    // Everything after the allocations runs in one body whose every exit
+    // (HPLAI_HIP returns from it) comes back here, so the buffers are freed
+    // on every error path.
     bool converged = false;
-    if (rc == HPLAI_OK) {
-        double hn[4];
-        rc = launch_norm_inf(n, B.rhs, B.norms + 0, s);
-        if (rc == HPLAI_OK)   // x and r are scratch until x0 is set
-            rc = lu.higham ? launch_higham_norm(n, lu.h_alpha, lu.h_beta, B.x, B.r, B.work, B.norms + 1, s)
-                           : launch_norm_inf(n, B.d, B.norms + 1, s);
-        if (rc == HPLAI_OK) rc = launch_vec_div(n, B.rhs, B.d, B.x, s);   // x0 = b ./ diag(A)
-        HPLAI_HIP(hipMemcpy(hn, B.norms, sizeof(double) * 2, hipMemcpyDeviceToHost));
-        // hpl_infnorm (panel_norm.hpp:78-84): 2 max d for the LCG matrix; the
-        // Higham matrix's largest row sum (higham_infnorm, panel_norm.hpp:87-135)
-        const double normb = hn[0], norma = lu.higham ? hn[1] : 2.0 * hn[1];
-        for (int it = 0; it < maxit && rc == HPLAI_OK; ++it) {
-            HPLAI_HIP(hipMemcpyAsync(B.r, B.rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            rc = launch_norm_inf(n, B.x, B.norms + 2, s);
-            const double dn2 = static_cast<double>(n) * static_cast<double>(n);
-            if (rc == HPLAI_OK)
-                rc = lu.timed(HPLAI_PHASE_IR_GEMV, 2.0 * dn2, s,
-                              [&] {
-                                  if (lu.higham)
-                                      return launch_higham_gemv(n, lu.h_alpha, lu.h_beta, -1.0, B.x, B.r, B.work, s);
-                                  return launch_otf_gemv(n, p->seed, B.d, -1.0, B.x, B.r, B.work, s);
-                              });
-            if (rc == HPLAI_OK) rc = launch_norm_inf(n, B.r, B.norms + 3, s);
-            if (rc) break;
-            HPLAI_HIP(hipMemcpy(hn + 2, B.norms + 2, sizeof(double) * 2, hipMemcpyDeviceToHost));
-            const double normx = hn[2], normr = hn[3];
-            const double h = normr / (norma * normx + normb) * 1. / (static_cast<double>(n) * DBL_EPSILON / 2);
-            res->trace_resid[it] = normr;
-            res->trace_harness[it] = h;
-            steps = it + 1;
-            if (p->verbose) {
-                printf("# iterative refinement: step=%3d, residual=%20.16e hpl-harness=%f\n", it, normr, h);
-                fflush(stdout);
-            }
-            if (h < 16.) {
-                res->residual = normr;
-                res->harness = h;
-                converged = true;
-                break;
-            }
-            rc = lu.timed(HPLAI_PHASE_IR_TRSV, 2.0 * dn2, s, [&] {
-                // diagonal blocks through the inverses the LU stored, where it stored them
-                if (tri_inverse_ok(b)) {
-                    const int64_t st = diag_inv_floats(b);
-                    int r = launch_trsv_lower_inv(n, b, B.A, lu.lda, B.r, B.inv, st, B.tsc, s);
-                    if (r == HPLAI_OK) r = launch_trsv_upper_inv(n, b, B.A, lu.lda, B.r, B.inv, st, B.tsc, s);
-                    return r;
-                }
-                int r = launch_trsv_lower(n, b, B.A, lu.lda, B.r, s);
-                if (r == HPLAI_OK) r = launch_trsv_upper(n, b, B.A, lu.lda, B.r, s);
-                return r;
-            });
-            if (rc == HPLAI_OK) rc = launch_vec_add(n, B.r, B.x, s);
-        }
-    }
-    HPLAI_HIP(hipDeviceSynchronize());
-    double t3 = now_s();
-    res->time_ir = t3 - t2;
-    res->time_total = t3 - t1;
-    if (p->profile) {
-        timer.collect(res);   // the IR phases recorded after the LU
-        res->phase_seconds[HPLAI_PHASE_TOTAL] = res->time_total;
-        double rest = res->time_total;
-        for (int i = 0; i < HPLAI_PHASE_TOTAL; ++i) rest -= res->phase_seconds[i];
-        res->phase_seconds[HPLAI_PHASE_MISC] = rest;
-    }
-    res->ir_steps = steps;
-    const double dn = static_cast<double>(n);
-    res->gflops = (2. / 3 * dn * dn * dn + 3. / 2 * dn * dn) / res->time_total * 1e-9;
-    if (!converged) {
-        res->residual = -1.0;
-        res->harness = -1.0;
-    }
-    if (rc == HPLAI_OK) {
-        rc = launch_checksum(n, B.A, lu.lda, B.chk, s);
+    int steps = 0;
+    auto body = [&]() -> int {
+        // ---- generation (main.cpp:256-275), outside the timed region ----
+        HPLAI_HIP(hipDeviceSynchronize());
+        double t0 = now_s();
         if (rc == HPLAI_OK)
-            HPLAI_HIP(hipMemcpy(res->checksum, B.chk, sizeof(double) * 3, hipMemcpyDeviceToHost));
-    }
+            rc = lu.higham ? launch_higham_diag(lu.h_alpha, lu.h_beta, 0, n, B.d, s)   // pdiaggen(HMGen)
+                           : launch_gen_diag(n, p->seed, 0, n, B.d, s);
+        if (lu.lazy) {
+            // pmatgen0 (matgen.hpp:80-105): partial sums start at zero
+            HPLAI_HIP(hipMemsetAsync(B.A, 0, sizeof(float) * lu.lda * n, s));
+            if (lu.trail16) HPLAI_HIP(hipMemsetAsync(B.P, 0, sizeof(uint16_t) * n * n, s));
+            if (lu.bback) HPLAI_HIP(hipMemsetAsync(B.Q, 0, sizeof(uint16_t) * n * n, s));
+        } else if (rc == HPLAI_OK) {
+            rc = lu.higham ? launch_gen_higham(lu.h_alpha, lu.h_beta, 0, 0, n, n, B.A, lu.lda, 0, s)
+                           : launch_gen_matrix(n, p->seed, 0, 0, n, n, B.d, B.A, lu.lda, 0, s);
+        }
+        if (rc == HPLAI_OK) rc = launch_gen_rhs(n, p->seed, 0, n, B.rhs, s);
+        HPLAI_HIP(hipDeviceSynchronize());
+        double t1 = now_s();
+        res->time_gen = t1 - t0;
+
+        // ---- LU ----
+        if (rc == HPLAI_OK) rc = lu.run(p->lookahead);
+        HPLAI_HIP(hipDeviceSynchronize());
+        epochs.flush(true);
+        double t2 = now_s();
+        res->time_lu = t2 - t1;
+        if (rc == HPLAI_OK) rc = timer.rc;
+        timer.collect(res);
+        if (rc == HPLAI_OK) rc = check_pivots(lu, B, "hplai_solve");
+
+        // ---- iterative refinement (iterative_refinement.hpp:18-57) ----
+        if (rc == HPLAI_OK) {
+            double hn[4];
+            rc = launch_norm_inf(n, B.rhs, B.norms + 0, s);
+            if (rc == HPLAI_OK)   // x and r are scratch until x0 is set
+                rc = lu.higham ? launch_higham_norm(n, lu.h_alpha, lu.h_beta, B.x, B.r, B.work, B.norms + 1, s)
+                               : launch_norm_inf(n, B.d, B.norms + 1, s);
+            if (rc == HPLAI_OK) rc = launch_vec_div(n, B.rhs, B.d, B.x, s);   // x0 = b ./ diag(A)
+            HPLAI_HIP(hipMemcpy(hn, B.norms, sizeof(double) * 2, hipMemcpyDeviceToHost));
+            // hpl_infnorm (panel_norm.hpp:78-84): 2 max d for the LCG matrix; the
+            // Higham matrix's largest row sum (higham_infnorm, panel_norm.hpp:87-135)
+            const double normb = hn[0], norma = lu.higham ? hn[1] : 2.0 * hn[1];
+            for (int it = 0; it < maxit && rc == HPLAI_OK; ++it) {
+                HPLAI_HIP(hipMemcpyAsync(B.r, B.rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+                rc = launch_norm_inf(n, B.x, B.norms + 2, s);
+                const double dn2 = static_cast<double>(n) * static_cast<double>(n);
+                if (rc == HPLAI_OK)
+                    rc = lu.timed(HPLAI_PHASE_IR_GEMV, 2.0 * dn2, s,
+                                  [&] {
+                                      if (lu.higham)
+                                          return launch_higham_gemv(n, lu.h_alpha, lu.h_beta, -1.0, B.x, B.r, B.work, s);
+                                      return launch_otf_gemv(n, p->seed, B.d, -1.0, B.x, B.r, B.work, s);
+                                  });
+                if (rc == HPLAI_OK) rc = launch_norm_inf(n, B.r, B.norms + 3, s);
+                if (rc) break;
+                HPLAI_HIP(hipMemcpy(hn + 2, B.norms + 2, sizeof(double) * 2, hipMemcpyDeviceToHost));
+                const double normx = hn[2], normr = hn[3];
+                const double h = normr / (norma * normx + normb) * 1. / (static_cast<double>(n) * DBL_EPSILON / 2);
+                res->trace_resid[it] = normr;
+                res->trace_harness[it] = h;
+                steps = it + 1;
+                if (p->verbose) {
+                    printf("# iterative refinement: step=%3d, residual=%20.16e hpl-harness=%f\n", it, normr, h);
+                    fflush(stdout);
+                }
+                if (h < 16.) {
+                    res->residual = normr;
+                    res->harness = h;
+                    converged = true;
+                    break;
+                }
+                rc = lu.timed(HPLAI_PHASE_IR_TRSV, 2.0 * dn2, s, [&] {
+                    // diagonal blocks through the inverses the LU stored, where it stored them
+                    if (tri_inverse_ok(b)) {
+                        const int64_t st = diag_inv_floats(b);
+                        int r = launch_trsv_lower_inv(n, b, B.A, lu.lda, B.r, B.inv, st, B.tsc, s);
+                        if (r == HPLAI_OK) r = launch_trsv_upper_inv(n, b, B.A, lu.lda, B.r, B.inv, st, B.tsc, s);
+                        return r;
+                    }
+                    int r = launch_trsv_lower(n, b, B.A, lu.lda, B.r, s);
+                    if (r == HPLAI_OK) r = launch_trsv_upper(n, b, B.A, lu.lda, B.r, s);
+                    return r;
+                });
+                if (rc == HPLAI_OK) rc = launch_vec_add(n, B.r, B.x, s);
+            }
+        }
+        HPLAI_HIP(hipDeviceSynchronize());
+        double t3 = now_s();
+        res->time_ir = t3 - t2;
+        res->time_total = t3 - t1;
+        if (p->profile) {
+            timer.collect(res);   // the IR phases recorded after the LU
+            res->phase_seconds[HPLAI_PHASE_TOTAL] = res->time_total;
+            double rest = res->time_total;
+            for (int i = 0; i < HPLAI_PHASE_TOTAL; ++i) rest -= res->phase_seconds[i];
+            res->phase_seconds[HPLAI_PHASE_MISC] = rest;
+        }
+        res->ir_steps = steps;
+        const double dn = static_cast<double>(n);
+        res->gflops = (2. / 3 * dn * dn * dn + 3. / 2 * dn * dn) / res->time_total * 1e-9;
+        if (!converged) {
+            res->residual = -1.0;
+            res->harness = -1.0;
+        }
+        if (rc == HPLAI_OK) {
+            rc = launch_checksum(n, B.A, lu.lda, B.chk, s);
+            if (rc == HPLAI_OK)
+                HPLAI_HIP(hipMemcpy(res->checksum, B.chk, sizeof(double) * 3, hipMemcpyDeviceToHost));
+        }
+        return rc;
+    };
+    rc = body();
+
     const bool keep = p->keep_factors != 0 && rc == HPLAI_OK;
     if (keep) {
         ctx->A = B.A;

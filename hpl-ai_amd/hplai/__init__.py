@@ -101,6 +101,7 @@ _SIGS = [
     ("hplai_solve", _int, [_vp, ctypes.POINTER(Params), ctypes.POINTER(Result)]),
     ("hplai_solution_x", _int, [_vp, _vp]),
     ("hplai_solution_factors", _int, [_vp, _vp, _i64]),
+    ("hplai_solution_block", _int, [_vp, _i64, _i64, _i64, _i64, _vp, _i64]),
     ("hplai_release", _int, [_vp]),
 ]
 
@@ -298,6 +299,13 @@ class Context:
         lu = np.zeros((n, n), dtype=np.float32)   # lu[j, i] = factor (i, j)
         check(lib().hplai_solution_factors(self.handle, lu.ctypes.data, n))
         return lu
+
+    def solution_block(self, i0, j0, m, k):
+        """The m x k block of the kept factors at (i0, j0), stored [col, row]."""
+        import numpy as np
+        blk = np.zeros((k, m), dtype=np.float32)
+        check(lib().hplai_solution_block(self.handle, i0, j0, m, k, blk.ctypes.data, m))
+        return blk
 
     def release(self):
         check(lib().hplai_release(self.handle))

@@ -311,6 +311,12 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *r);
  * the factors (n x n FP32, ld n) to host memory. */
 int hplai_solution_x(hplai_ctx ctx, double *x_host);
 int hplai_solution_factors(hplai_ctx ctx, float *lu_host, int64_t ldh);
+/* After hplai_solve with keep_factors: copy the m x k block of the factors at
+ * (i0, j0) to host memory (column-major, ldh >= m).  Lets a caller read the
+ * factors of a matrix too large to copy whole (155 GB at n = 196608); no
+ * reference counterpart (the reference's factors stay in its panels). */
+int hplai_solution_block(hplai_ctx ctx, int64_t i0, int64_t j0, int64_t m, int64_t k, float *host,
+                         int64_t ldh);
 int hplai_release(hplai_ctx ctx);   /* free buffers kept by keep_factors */
 
 #ifdef __cplusplus

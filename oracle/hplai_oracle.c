@@ -254,6 +254,14 @@ static int tile_exp(float amax, uint32_t mr_frac_bits, int er)
     }
     return er - E - (frac > mr_frac_bits ? 1 : 0);
 }
+/* e4m3 exponents become E8M0 block scales 127 - e of the scaled MFMA, so the
+ * library keeps them in [-127, 127]: a tile whose amax is below about 2^-119
+ * takes e = 127 and quantizes to (near) zero. */
+static int tile_exp_e4m3(float amax)
+{
+    int e = tile_exp(amax, 0x600000u, 8);
+    return e > 127 ? 127 : (e < -127 ? -127 : e);
+}
 #define E4M3_MR_FRAC 0x600000u   /* 1.75  */
 #define E4M3_ER 8
 #define F16_MR_FRAC  0x7fe000u   /* 1.9990234375 */
@@ -261,7 +269,7 @@ static int tile_exp(float amax, uint32_t mr_frac_bits, int er)
 
 int orc_tile_exp(int fmt, float amax)
 {
-    return fmt == ORC_LU_FP8 ? tile_exp(amax, E4M3_MR_FRAC, E4M3_ER) : tile_exp(amax, F16_MR_FRAC, F16_ER);
+    return fmt == ORC_LU_FP8 ? tile_exp_e4m3(amax) : tile_exp(amax, F16_MR_FRAC, F16_ER);
 }
 
 void orc_f32_to_e4m3_array(int64_t cnt, const float *x, uint8_t *q)
@@ -426,8 +434,7 @@ static void quant_tiles(int mode, int64_t m, int64_t k, int b, const float *src,
                     float v = fabsf(src[j * lds + i]);
                     if (v > amax) amax = v;
                 }
-            int e = mode == ORC_LU_FP8 ? tile_exp(amax, E4M3_MR_FRAC, E4M3_ER)
-                                       : tile_exp(amax, F16_MR_FRAC, F16_ER);
+            int e = mode == ORC_LU_FP8 ? tile_exp_e4m3(amax) : tile_exp(amax, F16_MR_FRAC, F16_ER);
             for (int64_t j = tj; j < tj + kj; ++j)
                 for (int64_t i = ti; i < ti + mi; ++i) {
                     float s = ldexpf(src[j * lds + i], e);
@@ -520,6 +527,104 @@ int orc_lu(int mode, int64_t n, int b, uint64_t seed, const float *dhigh,
     return 0;
 }
 
+/* 16-bit trailing storage: the reference's -d keeps the partial sums of the
+ * blocks that have not joined a panel in binary16 (the lower deck of the
+ * double-decker layout, panel.hpp:45-83), scaled by scalea = sqrt(n sqrt(n))
+ * (hpl_rand.hpp:154, panel_trf.hpp:717-719), and merges them when a block
+ * joins a panel, A = downscale * P + a_ij (LazyInitializer::update_diag /
+ * update, lazy_init.hpp:92-227).  This restates the library's form of it
+ * (its trail16 option): the products are summed in FP32 (the reference's
+ * hgemm rounds its accumulator to binary16 after every product,
+ * kernels/avx2.cpp:82-189), and each trailing update rounds once,
+ * P := fp16(P - scalea * sum), with the L panel quantized unscaled (binary16,
+ * qmode ORC_LU_PRECOMP) or with per-tile e4m3 exponents (ORC_LU_FP8).
+ * backbuf != 0 adds the compensated back buffer Q (back_buffer.hpp:81-170,
+ * bb_writeback_b_impl_fp16 back_buffer.cpp:32-55 in binary16 arithmetic):
+ * after every 32nd trailing update the blocks that do not join the next
+ * panel fold P into Q, Q := fl(Q + P/32), P := 32 (P/32 - (Q_new - Q)),
+ * and the merge uses x = P + 32 Q (lazy_init.hpp:229-329).  Binary16
+ * additions are computed in FP32 and rounded once, which equals binary16
+ * arithmetic (24 >= 2 * 11 + 2 significand bits). */
+static float h2f(uint16_t h) { return orc_f16_to_f32(h); }
+
+int orc_lu_trail16(int qmode, int backbuf, int64_t n, int b, uint64_t seed, const float *dhigh,
+                   float *A, int64_t lda, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    if (n % b || (qmode != ORC_LU_PRECOMP && qmode != ORC_LU_FP8)) return -1;
+    const int64_t nb = n / b;
+    float *Lq = (float *)malloc(sizeof(float) * (size_t)n * b);
+    float *Uq = (float *)malloc(sizeof(float) * (size_t)n * b);
+    float *T = (float *)malloc(sizeof(float) * (size_t)n * n);
+    uint16_t *P = (uint16_t *)calloc((size_t)n * n, sizeof(uint16_t));
+    uint16_t *Q = backbuf ? (uint16_t *)calloc((size_t)n * n, sizeof(uint16_t)) : NULL;
+    if (!Lq || !Uq || !T || !P || (backbuf && !Q)) {
+        free(Lq); free(Uq); free(T); free(P); free(Q);
+        return -2;
+    }
+    const float pscale = (float)sqrt((double)n * sqrt((double)n));
+    const float pdown = (float)(1.0 / (double)pscale);
+    for (int64_t k = 0; k < nb; ++k) {
+        const int64_t kk = k * b, e = kk + b, m = n - e;
+        /* merge: block column k (rows kk..n) and block row k (columns e..n) */
+        #pragma omp parallel for schedule(static)
+        for (int64_t j = kk; j < n; ++j) {
+            const int64_t ihi = j < e ? n : e;
+            uint64_t s = orc_lcg_state(seed, (uint64_t)kk + (uint64_t)n * (uint64_t)j);
+            for (int64_t i = kk; i < ihi; ++i) {
+                const float aij = (i == j) ? dhigh[i] : (float)orc_lcg_value(s);
+                float x = h2f(P[j * n + i]);
+                if (Q) x = fmaf(32.0f, h2f(Q[j * n + i]), x);
+                A[j * lda + i] = fmaf(pdown, x, aij);
+                s = LCG_A * s + LCG_C;
+            }
+        }
+        float *Akk = A + kk * lda + kk;
+        orc_getrf_nopiv_f32(b, Akk, lda);
+        if (m == 0) break;
+        float *L = Akk + b, *U = Akk + (int64_t)b * lda;
+        orc_trsm_runn_f32(m, b, Akk, lda, L, lda);
+        orc_trsm_llnu_f32(b, m, Akk, lda, U, lda);
+        if (qmode == ORC_LU_PRECOMP) {
+            for (int64_t j = 0; j < b; ++j) orc_round_f16_array(m, L + j * lda, Lq + j * m);
+            for (int64_t j = 0; j < m; ++j) orc_round_f16_array(b, U + j * lda, Uq + j * b);
+        } else {
+            quant_tiles(ORC_LU_FP8, m, b, b, L, lda, Lq, m);
+            quant_tiles(ORC_LU_FP8, b, m, b, U, lda, Uq, b);
+        }
+        /* T = -sum_l Lq Uq in FP32, then one binary16 rounding per element */
+        #pragma omp parallel for schedule(static)
+        for (int64_t j = 0; j < m; ++j) memset(T + j * m, 0, sizeof(float) * (size_t)m);
+        orc_schur_f32acc(m, m, b, Lq, m, Uq, b, T, m);
+        #pragma omp parallel for schedule(static)
+        for (int64_t j = 0; j < m; ++j)
+            for (int64_t i = 0; i < m; ++i) {
+                uint16_t *p = P + (e + j) * n + e + i;
+                *p = orc_f32_to_f16(fmaf(pscale, T[j * m + i], h2f(*p)));
+            }
+        if (Q && (k + 1) % 32 == 0 && e + b < n) {
+            const int64_t r1 = e + b;
+            #pragma omp parallel for schedule(static)
+            for (int64_t j = r1; j < n; ++j)
+                for (int64_t i = r1; i < n; ++i) {
+                    const float q0 = h2f(Q[j * n + i]);
+                    const float t1 = h2f(orc_f32_to_f16(h2f(P[j * n + i]) * h2f(orc_f32_to_f16(1.0f / 32.0f))));
+                    const float t2 = h2f(orc_f32_to_f16(q0 + t1));
+                    const float t3 = h2f(orc_f32_to_f16(t2 - q0));
+                    const float t4 = h2f(orc_f32_to_f16(t1 - t3));
+                    Q[j * n + i] = orc_f32_to_f16(t2);
+                    P[j * n + i] = orc_f32_to_f16(t4 * 32.0f);
+                }
+        }
+    }
+    free(Lq); free(Uq); free(T); free(P); free(Q);
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* Iterative refinement                                                */
 /* ------------------------------------------------------------------ */
@@ -530,6 +635,13 @@ int orc_lu(int mode, int64_t n, int b, uint64_t seed, const float *dhigh,
 void orc_otf_gemv(int64_t n, uint64_t seed, const double *d, double alpha,
                   const double *x, double *y, int nthreads)
 {
+    orc_otf_gemv_rows(n, seed, d, alpha, x, y, 0, n, nthreads);
+}
+
+/* The same for rows i0 .. i0+cnt-1 only (y indexed by global row). */
+void orc_otf_gemv_rows(int64_t n, uint64_t seed, const double *d, double alpha,
+                       const double *x, double *y, int64_t i0, int64_t cnt, int nthreads)
+{
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #else
@@ -537,7 +649,7 @@ void orc_otf_gemv(int64_t n, uint64_t seed, const double *d, double alpha,
 #endif
     coeff_t jn = coeff_pow(LCG1, (uint64_t)n);
     #pragma omp parallel for schedule(static, 16)
-    for (int64_t i = 0; i < n; ++i) {
+    for (int64_t i = i0; i < i0 + cnt; ++i) {
         uint64_t s = orc_lcg_state(seed, (uint64_t)i);
         double t0 = 0.0, t1 = 0.0;
         int64_t j = 0;

@@ -84,10 +84,20 @@ enum { ORC_LU_PRECOMP = 0,   /* FP32 C, FP16 panels, FP32 accumulate (no -d)    
 int orc_lu(int mode, int64_t n, int b, uint64_t seed, const float *dhigh,
            float *A, int64_t lda, int nthreads);
 
+/* The library's 16-bit trailing storage (lazy initialisation, partial sums
+ * of unfactored blocks in binary16 scaled by sqrt(n sqrt(n)), FP32-summed
+ * trailing products rounded once per update; backbuf: the compensated back
+ * buffer folded every 32 updates).  qmode: ORC_LU_PRECOMP (binary16 panels,
+ * no exponents) or ORC_LU_FP8.  A is written (regenerated lazily). */
+int orc_lu_trail16(int qmode, int backbuf, int64_t n, int b, uint64_t seed, const float *dhigh,
+                   float *A, int64_t lda, int nthreads);
+
 /* ---- IR (iterative_refinement.hpp:18-57) ---- */
 /* y += alpha * A x with A regenerated (panel_gemv.hpp:16-83, otf_gemv.cpp) */
 void   orc_otf_gemv(int64_t n, uint64_t seed, const double *d, double alpha,
                     const double *x, double *y, int nthreads);
+void   orc_otf_gemv_rows(int64_t n, uint64_t seed, const double *d, double alpha,
+                         const double *x, double *y, int64_t i0, int64_t cnt, int nthreads);
 void   orc_trsv_lower_unit(int64_t n, const float *lu, int64_t lda, double *x); /* panel_trsv.hpp:115-124 */
 void   orc_trsv_upper(int64_t n, const float *lu, int64_t lda, double *x);      /* panel_trsv.hpp:142-153 */
 double orc_infnorm(int64_t n, const double *x);                                 /* panel_norm.hpp:8-19    */

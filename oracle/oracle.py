@@ -67,7 +67,10 @@ def lib():
         L.orc_schur_f16acc.argtypes = [_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64]
         L.orc_lu.argtypes = [_int, _i64, _int, _u64, _vp, _vp, _i64, _int]
         L.orc_lu.restype = _int
+        L.orc_lu_trail16.argtypes = [_int, _int, _i64, _int, _u64, _vp, _vp, _i64, _int]
+        L.orc_lu_trail16.restype = _int
         L.orc_otf_gemv.argtypes = [_i64, _u64, _vp, _dbl, _vp, _vp, _int]
+        L.orc_otf_gemv_rows.argtypes = [_i64, _u64, _vp, _dbl, _vp, _vp, _i64, _i64, _int]
         L.orc_trsv_lower_unit.argtypes = [_i64, _vp, _i64, _vp]
         L.orc_trsv_upper.argtypes = [_i64, _vp, _i64, _vp]
         L.orc_infnorm.argtypes = [_i64, _vp]
@@ -152,6 +155,20 @@ def lu(mode, n, b, seed=SEED, A=None, d=None, nthreads=0):
     return A, d
 
 
+def lu_trail16(qmode, backbuf, n, b, seed=SEED, d=None, nthreads=0):
+    """The library's 16-bit trailing storage (lazy initialisation; qmode
+    LU_PRECOMP for binary16 panels or LU_FP8), with the back buffer when
+    backbuf.  Returns the FP32 LU array [col, row] and the FP64 diagonal."""
+    if d is None:
+        d = gen_diag(n, seed)
+    A = np.zeros((n, n), dtype=np.float32)
+    dhigh = d.astype(np.float32)
+    rc = lib().orc_lu_trail16(qmode, 1 if backbuf else 0, n, b, seed, _ptr(dhigh), _ptr(A), n, nthreads)
+    if rc != 0:
+        raise RuntimeError("orc_lu_trail16 failed: %d" % rc)
+    return A, d
+
+
 def ir(LU, n, d, rhs=None, seed=SEED, maxit=50, nthreads=0):
     rhs = gen_rhs(n, seed) if rhs is None else rhs
     x = np.zeros(n, dtype=np.float64)
@@ -174,6 +191,14 @@ def otf_gemv(n, d, alpha, x, y, seed=SEED, nthreads=0):
     y = np.array(y, dtype=np.float64, copy=True)
     lib().orc_otf_gemv(n, seed, _ptr(d), alpha, _ptr(np.ascontiguousarray(x, dtype=np.float64)),
                        _ptr(y), nthreads)
+    return y
+
+
+def otf_gemv_rows(n, d, alpha, x, y, i0, cnt, seed=SEED, nthreads=0):
+    """Rows i0 .. i0+cnt-1 of y + alpha A x (A regenerated); d and x full length."""
+    y = np.array(y, dtype=np.float64, copy=True)
+    lib().orc_otf_gemv_rows(n, seed, _ptr(np.ascontiguousarray(d, dtype=np.float64)), alpha,
+                            _ptr(np.ascontiguousarray(x, dtype=np.float64)), _ptr(y), i0, cnt, nthreads)
     return y
 
 

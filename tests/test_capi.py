@@ -6,6 +6,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -80,3 +81,38 @@ def test_driver_help():
     out = subprocess.run([hplai.DRIVER_PATH, "1200", "60", "2", "--help"], capture_output=True,
                          text=True, timeout=60)
     assert "usage: ./driver.out <n> <b> <P>" in out.stdout
+
+
+def test_e4m3_exponent_clamp(tmp_path, oracle):
+    """e4m3 tile exponents become E8M0 block scales 127 - e, so the library
+    keeps them in [-127, 127] (device_common.hpp, host code compiled here; no
+    GPU call): tiles below ~2^-119 take e = 127 instead of a NaN scale byte,
+    and the oracle clamps the same way."""
+    src = tmp_path / "clamp.cpp"
+    src.write_text(r'''
+#include "device_common.hpp"
+#include <cstdio>
+int main()
+{
+    const float xs[] = {1.0f, 448.0f, 449.0f, 0x1p-119f, 0x1p-120f, 1e-40f, 0.0f};
+    for (float x : xs)
+        std::printf("%a %d %d\n", x, hplai::e4m3_exponent(x), hplai::e8m0_scale(hplai::e4m3_exponent(x)));
+    std::printf("%d %d\n", hplai::e8m0_scale(200), hplai::e8m0_scale(-200));
+    return 0;
+}
+''')
+    exe = tmp_path / "clamp"
+    hipcc = "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O1", "-I",
+                    os.path.join(ROOT, "hpl-ai_amd", "csrc"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    rows = [ln.split() for ln in out[:7]]
+    got = {float.fromhex(r[0]): (int(r[1]), int(r[2])) for r in rows}
+    assert got[1.0] == (8, 119) and got[448.0] == (0, 127) and got[449.0] == (-1, 128)
+    assert got[2.0 ** -119] == (127, 0) and got[2.0 ** -120] == (127, 0)
+    assert got[float(np.float32(1e-40))] == (127, 0) and got[0.0] == (0, 127)
+    assert out[7].split() == ["0", "254"]
+    for x, (e, _) in got.items():
+        assert oracle.tile_exp(2, x) == e, x
