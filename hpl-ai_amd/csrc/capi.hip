@@ -21,6 +21,9 @@ void set_error(const char *fmt, ...)
 int hip_fail(hipError_t e, const char *what)
 {
     set_error("%s: %s", what, hipGetErrorString(e));
+    // the runtime keeps the last error until it is read: clear it, so that a
+    // failed allocation is not reported again by the next call's launch check
+    (void)hipGetLastError();
     return e == hipErrorOutOfMemory ? HPLAI_ENOMEM : HPLAI_EHIP;
 }
 
@@ -182,6 +185,25 @@ int hplai_schur_update(hplai_ctx ctx, int fmt, int64_t m, int64_t n, int nslices
                 "schur_update: bad shape/pointers");
     return launch_schur(fmt, m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu, tb, C, ldc,
                         ctx->num_cu, as_stream(stream));
+}
+
+int hplai_gemmschur_f16(hplai_ctx ctx, int64_t n, int64_t m, int64_t k, const uint16_t *a, int64_t lda,
+                        const uint16_t *b, int64_t ldb, float *c, int64_t ldc, void *stream)
+{
+    CTX_CHECK(ctx);
+    HPLAI_CHECK(n >= 0 && m >= 0 && k >= 0, "gemmschur: negative size (n=%lld m=%lld k=%lld)", (long long)n,
+                (long long)m, (long long)k);
+    if (n == 0 || m == 0 || k == 0) return HPLAI_OK;
+    HPLAI_CHECK(a != nullptr && b != nullptr && c != nullptr, "gemmschur: NULL operand");
+    HPLAI_CHECK(lda >= n && ldb >= k && ldc >= n, "gemmschur: need lda >= n, ldb >= k, ldc >= n");
+    HPLAI_CHECK(k <= (int64_t(1) << 24), "gemmschur: k=%lld too large", (long long)k);
+    hipStream_t s = as_stream(stream);
+    void *scratch = nullptr;
+    HPLAI_HIP(hipMallocAsync(&scratch, static_cast<size_t>(gemmschur_scratch_bytes(n, m, k)), s));
+    const int rc = launch_gemmschur_f16(n, m, k, a, lda, b, ldb, c, ldc, static_cast<uint16_t *>(scratch),
+                                        ctx->num_cu, s);
+    HPLAI_HIP(hipFreeAsync(scratch, s));
+    return rc;
 }
 
 int hplai_schur_update_f16c(hplai_ctx ctx, int fmt, int64_t m, int64_t n, int nslices, int kslice,

@@ -96,6 +96,13 @@ int launch_schur_c16(int fmt, int64_t m, int64_t n, int nslices, int kslice, con
 // true when launch_schur runs the streamed 256 x 256 kernel for this shape
 bool schur_large_tiles(int fmt, int64_t m, int64_t n, int tb, int num_cu);
 
+// the reference's column-major gemmschur seam (gemmschur.hip): both operands
+// staged K-contiguous with K padded to 64 in `scratch`
+// (gemmschur_scratch_bytes), then C(n x m) -= A B with FP32 accumulation
+int64_t gemmschur_scratch_bytes(int64_t n, int64_t m, int64_t k);
+int launch_gemmschur_f16(int64_t n, int64_t m, int64_t k, const uint16_t *a, int64_t lda, const uint16_t *b,
+                         int64_t ldb, float *c, int64_t ldc, uint16_t *scratch, int num_cu, hipStream_t s);
+
 int launch_otf_gemv(int64_t n, uint64_t seed, const double *d, double alpha, const double *x,
                     double *y, double *work, hipStream_t s);
 int64_t otf_gemv_work_doubles(int64_t n);

@@ -79,6 +79,7 @@ _SIGS = [
     ("hplai_quantize", _int, [_vp, _int, _int, _i64, _i64, _int, _vp, _i64, _vp, _i64, _vp, _vp]),
     ("hplai_schur_update", _int, [_vp, _int, _i64, _i64, _int, _int, _vp, _i64, _vp, _i64, _vp, _i64,
                                   _vp, _i64, _int, _vp, _i64, _vp]),
+    ("hplai_gemmschur_f16", _int, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
     ("hplai_schur_update_f16c", _int, [_vp, _int, _i64, _i64, _int, _int, _vp, _i64, _vp, _i64, _vp, _i64,
                                        _vp, _i64, _int, _vp, _i64, ctypes.c_float, _vp]),
     ("hplai_otf_gemv_f64", _int, [_vp, _i64, _u64, _vp, _dbl, _vp, _vp, _vp]),
@@ -217,6 +218,12 @@ class Context:
         check(lib().hplai_schur_update(self.handle, fmt, m, n, nslices, kslice, ptr(Lq), ldl, ptr(eL),
                                        ldel, ptr(Uq), ldu, ptr(eU), ldeu, tb, ptr(C), ldc,
                                        stream_ptr(stream)))
+
+    def gemmschur_f16(self, n, m, k, a, lda, b, ldb, c, ldc, stream=None):
+        """The reference's gemmschur<fp16, float>: C(n x m) -= A(n x k) B(k x m),
+        column-major binary16 operands, FP32 C."""
+        check(lib().hplai_gemmschur_f16(self.handle, n, m, k, ptr(a), lda, ptr(b), ldb, ptr(c), ldc,
+                                        stream_ptr(stream)))
 
     def schur_update_f16c(self, fmt, m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu, tb, C,
                           ldc, cscale, stream=None):

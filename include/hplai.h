@@ -103,6 +103,17 @@ int hplai_schur_update(hplai_ctx ctx, int fmt, int64_t m, int64_t n, int nslices
                        const void *Lq, int64_t ldl, const int32_t *eL, int64_t ldel,
                        const void *Uq, int64_t ldu, const int32_t *eU, int64_t ldeu, int tb,
                        float *C, int64_t ldc, void *stream);
+/* Replaces gemmschur(int n, int m, int k, const fp16 *a, int lda, const fp16 *b, int ldb,
+ * float *c, int ldc) (schur_updator.hpp:28-31 -> shgemm fp16sim.hpp:217-233) with the
+ * reference's own operand layout: C(n x m) -= A(n x k) B(k x m), all column-major,
+ * A = the L panel as convert_left_panels writes it (column l at a + l*lda,
+ * panel_trf.hpp:182-207), B = the U panel of convert_right_panels (column j at
+ * b + j*ldb, panel_trf.hpp:208-233), binary16 bits; FP32 accumulation, C FP32
+ * (device).  Any k >= 0 (the reference's C1 has k = b = 60).  The operands are
+ * staged K-contiguous in a stream-ordered scratch allocation (hipMallocAsync,
+ * (n + m) * roundup(k, 64) * 2 bytes) freed on the same stream. */
+int hplai_gemmschur_f16(hplai_ctx ctx, int64_t n, int64_t m, int64_t k, const uint16_t *a, int64_t lda,
+                        const uint16_t *b, int64_t ldb, float *c, int64_t ldc, void *stream);
 /* Replaces the FP16-C overload gemmschur(fp16 a, fp16 b, fp16 c) (schur_updator.hpp:24-27,
  * hgemm with alpha = -1): C := fp16(C - cscale * 2^-(eL+eU) Lq Uq) with C an FP16
  * (binary16 bits) column-major m x n array, FP32 accumulation (the reference rounds

@@ -116,3 +116,29 @@ int main()
     assert out[7].split() == ["0", "254"]
     for x, (e, _) in got.items():
         assert oracle.tile_exp(2, x) == e, x
+
+
+def test_seam_example_links_and_reports_without_gpu(tmp_path):
+    """bin/seam_example (the reference's gemmschur bound from plain C, built
+    with gcc) loads the library; with no GPU its hplai_create call returns an
+    error, which it reports instead of crashing."""
+    import hplai
+    exe = os.path.join(os.path.dirname(hplai.DRIVER_PATH), "seam_example")
+    if not os.path.exists(exe):
+        pytest.skip("seam_example not built")
+    fin = tmp_path / "in.bin"
+    np.array([2, 2, 1, 2, 1, 2], dtype=np.int32).tofile(fin)
+    with open(fin, "ab") as f:
+        np.zeros(2, np.uint16).tofile(f)
+        np.zeros(2, np.uint16).tofile(f)
+        np.zeros(4, np.float32).tofile(f)
+    out = subprocess.run([exe, str(fin), str(tmp_path / "out.bin")], capture_output=True, text=True, timeout=60)
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        assert out.returncode == 0, out.stderr
+    else:
+        assert out.returncode == 1 and "seam_example" in out.stderr

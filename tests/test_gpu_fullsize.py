@@ -13,7 +13,7 @@ take hours to days), so the solve is checked by what does not need it:
   - the step-0 residual b - A x0, x0 = b ./ d (iterative_refinement.hpp:33-41),
     computed through hplai_otf_gemv_f64 equals the solve's first trace entry
     and the oracle's full FP64 residual (orc_otf_gemv, A regenerated on the
-    CPU) within 1e-13;
+    CPU) within the FP64 summation-order bound 8 sqrt(n) 2^-53 sum_j |a_ij x_j|;
   - sampled blocks of the kept factors reproduce the matrix from the
     quantized operands the Schur updates consumed (parity.block_residual):
     first blocks, blocks on both sides of an outer-block boundary, the
@@ -68,9 +68,15 @@ def test_fullsize_solve(ctx, oracle, name, n, b, fmt, outer, max_steps):
         r0 = float(np.max(np.abs(rg)))
         assert abs(r0 - res["trace_resid"][0]) <= 1e-14 * r0, (r0, res["trace_resid"][0])
         rr = oracle.otf_gemv(n, dh, -1.0, x0, bh)
-        rel = np.max(np.abs(rg - rr)) / np.max(np.abs(rr))
-        print("  step-0 residual %.16e, vs oracle %.3g" % (r0, rel), flush=True)
-        assert rel < 1e-13
+        # both are FP64 sums of n terms in different orders: |dr_i| within
+        # 8 sqrt(n) 2^-53 sum_j |a_ij x_j|, and sum_j |a_ij x_j| <= 2 d_i max|x0|
+        # (d_i is the row's off-diagonal absolute sum).  r is ~100x smaller
+        # than the terms it cancels, so a relative 1e-13 does not hold here.
+        scale = 8.0 * np.sqrt(n) * 2.0 ** -53 * 2.0 * dh * np.max(np.abs(x0))
+        ratio = float(np.max(np.abs(rg - rr) / scale))
+        print("  step-0 residual %.16e, vs oracle: %.3g relative, %.3g of the bound" %
+              (r0, np.max(np.abs(rg - rr)) / np.max(np.abs(rr)), ratio), flush=True)
+        assert ratio < 1.0
 
         # sampled-block factor residual
         nb = n // b

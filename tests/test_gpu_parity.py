@@ -617,3 +617,68 @@ def test_driver_delayed_trailing_storage(tmp_path, extra, trailing):
     assert ("trailing=%s" % trailing) in out.stdout and "lazy" in out.stdout
     m = re.search(r"hpl-harness = ([0-9.]+)$", out.stdout, re.M)
     assert m and float(m.group(1)) < 16.0
+
+
+# ------------------------------------------------------------------ the reference's gemmschur seam
+def _seam_run(tmp_path, a, lda, b, ldb, c, ldc, n, m, k):
+    """Run bin/seam_example (plain C, gcc, bound as INTEGRATION.md shows)."""
+    import os
+    import subprocess
+    import hplai
+    exe = os.path.join(os.path.dirname(hplai.DRIVER_PATH), "seam_example")
+    assert os.path.exists(exe), "seam_example not built"
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([n, m, k, lda, ldb, ldc], dtype=np.int32).tofile(f)
+        np.ascontiguousarray(a, dtype=np.uint16).tofile(f)
+        np.ascontiguousarray(b, dtype=np.uint16).tofile(f)
+        np.ascontiguousarray(c, dtype=np.float32).tofile(f)
+    out = subprocess.run([exe, str(fin), str(fout)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    return np.fromfile(fout, dtype=np.float32).reshape(c.shape)
+
+
+def _seam_tol(a16, lda, b16, ldb, c, n, m, k):
+    """4 K 2^-24 (|A||B|) + 2^-23 |C| elementwise ([col, row] arrays)."""
+    A = np.abs(a16.view(np.float16).astype(np.float64).reshape(k, lda)[:, :n])
+    Bm = np.abs(b16.view(np.float16).astype(np.float64).reshape(m, ldb)[:, :k])
+    ab = Bm @ A                                   # [col j, row i] = sum_l |a_il||b_lj|
+    return 4 * k * 2.0 ** -24 * ab + 2.0 ** -23 * np.abs(c[:, :n].astype(np.float64))
+
+
+def test_gemmschur_seam_reference_fixture(tmp_path):
+    """The reference's own shgemm_opt output (kernels/avx2.cpp:249-302, built
+    from the reference's sources into oracle/_ref; fixture written by
+    tests/golden/make_golden.py): C(53 x 40) -= A(53 x 150) B(150 x 40), the
+    reference's column-major operands passed straight through the seam."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_kernels.npz"), allow_pickle=False)
+    gm, gn, gk = (int(v) for v in fx["gemm_shape"])
+    got = _seam_run(tmp_path, fx["gemm_a"], gm, fx["gemm_b"], gk, fx["gemm_c32"], gm, gm, gn, gk)
+    tol = _seam_tol(fx["gemm_a"], gm, fx["gemm_b"], gk, fx["gemm_c32"], gm, gn, gk)
+    err = np.abs(got.astype(np.float64) - fx["shgemm_out"])
+    assert np.all(err <= tol), float(np.max(err / tol))
+
+
+@pytest.mark.parametrize("n,m,k,lda,ldb,ldc", [(540, 540, 60, 704, 60, 704),    # C1 2x2 local, lda 704
+                                               (1140, 1140, 60, 1200, 64, 1200),  # C1 1x1, step 0
+                                               (300, 77, 1, 300, 1, 301), (129, 255, 65, 129, 70, 130)])
+def test_gemmschur_seam_vs_oracle(tmp_path, oracle, n, m, k, lda, ldb, ldc):
+    """C1 shapes (k = b = 60, not a multiple of the 64-value stage) and ragged
+    ones, bound from C exactly as INTEGRATION.md shows, against the oracle's
+    shgemm restatement (bit-exact with the reference's shgemm_opt,
+    test_oracle.py) within the FP32-summation-order bound."""
+    rng = np.random.default_rng(n + m + k)
+    a = oracle.f32_to_f16_bits(rng.uniform(-2e-3, 2e-3, size=(k, lda)).astype(np.float32))
+    b = oracle.f32_to_f16_bits(rng.uniform(-0.5, 0.5, size=(m, ldb)).astype(np.float32))
+    c = rng.uniform(-1, 1, size=(m, ldc)).astype(np.float32)
+    got = _seam_run(tmp_path, a, lda, b, ldb, c, ldc, n, m, k)
+    af = a.view(np.float16).astype(np.float32)
+    bf = b.view(np.float16).astype(np.float32)
+    ref = c.copy()
+    oracle.lib().orc_schur_f32acc(n, m, k, oracle._ptr(af), lda, oracle._ptr(bf), ldb, oracle._ptr(ref), ldc)
+    tol = _seam_tol(a, lda, b, ldb, c, n, m, k)
+    err = np.abs(got[:, :n].astype(np.float64) - ref[:, :n])
+    assert np.all(err <= tol), float(np.max(err / tol))
+    # rows past n in each column are not touched
+    assert np.array_equal(got[:, n:], c[:, n:])
