@@ -65,6 +65,14 @@ def run_form(ctx, args, fmt, outer):
            ("schur_launches", "schur_flops", "schur_ms", "schur_large_launches", "schur_large_flops",
             "schur_large_ms", "schur_large_bytes", "schur_inner_launches", "schur_inner_flops",
             "schur_inner_ms")}
+    classes = {}
+    for name in results[0]["schur_classes"]:
+        nl = sum(r["schur_classes"][name][0] for r in results)
+        fl = sum(r["schur_classes"][name][1] for r in results)
+        ms = sum(r["schur_classes"][name][2] for r in results)
+        classes[name] = {"launches": nl // len(results), "ms_per_solve": ms / len(results),
+                         "tflops": fl / (ms * 1e-3) * 1e-12 if ms > 0 else None}
+    tot["schur_classes"] = classes
     return dict(fmt=fmt, outer=outer, time_s=t, wall_s=wall, tflops=hpl_flops(args.n) / t * 1e-12,
                 time_lu_s=res["time_lu"], time_ir_s=res["time_ir"], time_gen_s=res["time_gen"],
                 ir_steps=res["ir_steps"], harness=res["harness"], residual=res["residual"],
@@ -100,6 +108,7 @@ def roofline(p, fmt, n):
             "algorithmic_bytes_per_launch": p["schur_large_bytes"] / nl,
             "all_schur_launches": {"launches": p["schur_launches"], "tflops": p["schur_tflops"],
                                    "ms": p["schur_ms"]},
+            "by_class": p["schur_classes"],
             "in_block_launches": {"launches": p["schur_inner_launches"], "ms": p["schur_inner_ms"],
                                    "tflops": (p["schur_inner_flops"] / (p["schur_inner_ms"] * 1e-3) * 1e-12)
                                    if p["schur_inner_ms"] > 0 else None}}
@@ -177,11 +186,26 @@ def bench_batch(args):
     for _ in range(max(args.warmup, 1)):
         bt.run("graph")
     runs = [bt.run("graph") for _ in range(args.steps)]
-    base = bt.run("per_system")
+    _, steps, _, _ = bt.results()
+    baselines = {m: bt.run(m) for m in ("per_system", "per_system_4streams", "grid_per_system")}
+    base = baselines["per_system"]
     bt.close()
     ctx.close()
     t = sum(r["seconds"] for r in runs) / len(runs)
     last = runs[-1]
+    # roofline of batch_kernel: HBM-bound.  Algorithmic bytes per system: the
+    # LU's trailing FP32 matrix read and written at every panel step,
+    # sum_k 8 (n - k nb)^2 ~ 8 n^3 / (3 nb); the generation's FP64 and FP32
+    # copies, 12 n^2; one pass over the FP64 matrix per refinement residual,
+    # 8 n^2 per step.  The kernel is the whole timed launch.
+    bounds = (128, 256, 512)
+    nbytes = 0.0
+    for q in range(args.batch_count):
+        nq = float(n[q])
+        nb = table[sum(1 for x in bounds if n[q] >= x)]
+        trail = sum(8.0 * (nq - k * nb) ** 2 for k in range(int(n[q]) // nb + 1) if nq - k * nb > 0)
+        nbytes += trail + 12.0 * nq * nq + 8.0 * nq * nq * max(int(v) for v in steps[q][:int(nrhs[q])])
+    achieved = nbytes / t * 1e-9
     line = {
         "metric": "batched HPL-AI solves: systems/s, one persistent hipGraph launch, 1 GPU",
         "value": args.batch_count / t, "unit": "systems/s", "n_gpus": 1, "steps": args.steps,
@@ -191,9 +215,14 @@ def bench_batch(args):
                    "autotuned_nb_per_bucket": table},
         "p50_us": last["p50_us"], "p99_us": last["p99_us"], "max_us": last["max_us"],
         "failed": last["failed"], "max_harness": last["max_harness"], "max_ir_steps": last["max_steps"],
-        "per_system_loop": {"systems_per_s": base["systems_per_s"], "seconds": base["seconds"],
-                            "p50_us": base["p50_us"], "p99_us": base["p99_us"], "failed": base["failed"]},
+        "baselines": {m: {"systems_per_s": r["systems_per_s"], "seconds": r["seconds"], "p50_us": r["p50_us"],
+                          "p99_us": r["p99_us"], "failed": r["failed"]} for m, r in baselines.items()},
         "speedup_vs_per_system_loop": base["seconds"] / t,
+        "speedup_vs_per_system_4streams": baselines["per_system_4streams"]["seconds"] / t,
+        "speedup_vs_grid_per_system": baselines["grid_per_system"]["seconds"] / t,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_GBS, "traffic": None, "kernel": "batch_kernel",
+                     "algorithmic_bytes": nbytes},
     }
     print(json.dumps(line))
 

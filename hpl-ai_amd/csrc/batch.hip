@@ -580,7 +580,9 @@ struct hplai_batch_s {
     std::vector<int32_t> order;           // largest first
     int32_t nb_table[kBuckets] = {32, 32, 32, 32};
     BatchSys *d_sys = nullptr;
-    int32_t *d_order = nullptr, *d_counter = nullptr;
+    int32_t *d_order = nullptr, *d_counter = nullptr;   // d_counter[0]: one launch; [1 + s]: system s alone
+    int32_t *d_ident = nullptr;                          // systems in input order
+    hipStream_t streams[4] = {};                          // baseline: per-system launches on 4 streams
     float *fpool = nullptr;
     double *dpool = nullptr;
     BatchOut *d_out = nullptr;
@@ -608,11 +610,13 @@ int upload_sys(hplai_batch b)
     return HPLAI_OK;
 }
 
-int launch_batch(hplai_batch b, const int32_t *d_order, int count, int grid, int maxit, hipStream_t s)
+int launch_batch(hplai_batch b, const int32_t *d_order, int count, int grid, int maxit, hipStream_t s,
+                 int32_t *counter = nullptr)
 {
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, s, b->d_counter);
+    if (counter == nullptr) counter = b->d_counter;
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, s, counter);
     hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(kBT), kBatchLds, s, b->d_sys, d_order, count, b->fpool,
-                       b->dpool, b->d_out, b->d_counter, maxit);
+                       b->dpool, b->d_out, counter, maxit);
     HPLAI_HIP(hipGetLastError());
     return HPLAI_OK;
 }
@@ -674,13 +678,23 @@ int hplai_batch_create(hplai_ctx ctx, int count, const int32_t *n, const int32_t
     };
     alloc(reinterpret_cast<void **>(&b->d_sys), sizeof(BatchSys) * count);
     alloc(reinterpret_cast<void **>(&b->d_order), sizeof(int32_t) * count);
-    alloc(reinterpret_cast<void **>(&b->d_counter), sizeof(int32_t));
+    alloc(reinterpret_cast<void **>(&b->d_counter), sizeof(int32_t) * (count + 1));
+    alloc(reinterpret_cast<void **>(&b->d_ident), sizeof(int32_t) * count);
     alloc(reinterpret_cast<void **>(&b->fpool), sizeof(float) * fo);
     alloc(reinterpret_cast<void **>(&b->dpool), sizeof(double) * dof);
     alloc(reinterpret_cast<void **>(&b->d_out), sizeof(BatchOut) * count);
     if (rc == HPLAI_OK &&
         hipMemcpy(b->d_order, b->order.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice) != hipSuccess)
         rc = hip_fail(hipGetLastError(), "hplai_batch_create: copy");
+    if (rc == HPLAI_OK) {
+        std::vector<int32_t> ident(count);
+        for (int q = 0; q < count; ++q) ident[q] = q;
+        if (hipMemcpy(b->d_ident, ident.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice) != hipSuccess)
+            rc = hip_fail(hipGetLastError(), "hplai_batch_create: copy");
+    }
+    for (int q = 0; q < 4 && rc == HPLAI_OK; ++q)
+        if (hipStreamCreateWithFlags(&b->streams[q], hipStreamNonBlocking) != hipSuccess)
+            rc = hip_fail(hipGetLastError(), "hplai_batch_create: streams");
     if (rc == HPLAI_OK) rc = upload_sys(b);
     if (rc == HPLAI_OK &&
         hipFuncSetAttribute(reinterpret_cast<const void *>(batch_kernel),
@@ -707,6 +721,9 @@ int hplai_batch_destroy(hplai_batch b)
     (void)hipFree(b->d_sys);
     (void)hipFree(b->d_order);
     (void)hipFree(b->d_counter);
+    (void)hipFree(b->d_ident);
+    for (int q = 0; q < 4; ++q)
+        if (b->streams[q]) (void)hipStreamDestroy(b->streams[q]);
     (void)hipFree(b->fpool);
     (void)hipFree(b->dpool);
     (void)hipFree(b->d_out);
@@ -735,7 +752,7 @@ int hplai_batch_get_nb(hplai_batch b, int32_t nb[4])
 int hplai_batch_run(hplai_batch b, int mode, int maxit, hplai_batch_stats *st)
 {
     HPLAI_CHECK(b != nullptr && st != nullptr, "hplai_batch_run: NULL argument");
-    HPLAI_CHECK(mode >= 0 && mode <= 2, "hplai_batch_run: mode %d", mode);
+    HPLAI_CHECK(mode >= 0 && mode <= 4, "hplai_batch_run: mode %d", mode);
     if (maxit <= 0) maxit = 50;
     HPLAI_HIP(hipSetDevice(b->ctx->device));
     hipStream_t s = b->ctx->side;
@@ -766,10 +783,20 @@ int hplai_batch_run(hplai_batch b, int mode, int maxit, hplai_batch_stats *st)
             }
         }
         if (rc == HPLAI_OK) HPLAI_HIP(hipGraphLaunch(b->exec, s));
-    } else {
+    } else if (mode == 2) {
         // baseline: one launch (one workgroup) per system, in the same order
         for (int q = 0; q < b->count && rc == HPLAI_OK; ++q)
             rc = launch_batch(b, b->d_order + q, 1, 1, maxit, s);
+    } else if (mode == 3) {
+        // baseline: one launch per system, round-robin over four streams (the
+        // process has four hardware queues), each launch with its own counter
+        for (int q = 0; q < b->count && rc == HPLAI_OK; ++q)
+            rc = launch_batch(b, b->d_order + q, 1, 1, maxit, b->streams[q & 3], b->d_counter + 1 + q);
+        for (int q = 0; q < 4; ++q) HPLAI_HIP(hipStreamSynchronize(b->streams[q]));
+    } else {
+        // baseline: one launch of one workgroup per system in input order (a
+        // plain batched grid: no largest-first order, no persistent workgroups)
+        rc = launch_batch(b, b->d_ident, b->count, b->count, maxit, s);
     }
     HPLAI_HIP(hipStreamSynchronize(s));
     const auto t1 = std::chrono::steady_clock::now();

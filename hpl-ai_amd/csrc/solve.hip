@@ -95,7 +95,7 @@ struct GemmTimer {
     bool on = false;
     std::vector<hipEvent_t> ev;   // start/stop pairs
     std::vector<double> flops, bytes;
-    std::vector<char> large, inner;
+    std::vector<char> large, inner, cls;
     std::vector<hipEvent_t> pev;  // phase start/stop pairs
     std::vector<int> pitem;
     std::vector<double> pops;
@@ -119,7 +119,7 @@ struct GemmTimer {
         pitem.push_back(item);
         pops.push_back(ops);
     }
-    void begin(hipStream_t s, double f, double by, bool is_large, bool is_inner)
+    void begin(hipStream_t s, double f, double by, bool is_large, bool is_inner, int c)
     {
         if (!on || rc) return;
         hipEvent_t a = nullptr, b = nullptr;
@@ -133,6 +133,7 @@ struct GemmTimer {
         bytes.push_back(by);
         large.push_back(is_large ? 1 : 0);
         inner.push_back(is_inner ? 1 : 0);
+        cls.push_back(static_cast<char>(c));
         (void)hipEventRecord(a, s);
     }
     void end(hipStream_t s)
@@ -147,11 +148,19 @@ struct GemmTimer {
         for (int i = 0; i < HPLAI_PHASES; ++i) r->phase_seconds[i] = r->phase_ops[i] = 0.0;
         double ms = 0.0, fl = 0.0, lms = 0.0, lfl = 0.0, lby = 0.0, ims = 0.0, ifl = 0.0;
         int64_t ln = 0, in = 0;
+        for (int c = 0; c < 4; ++c) {
+            r->schur_class_launches[c] = 0;
+            r->schur_class_flops[c] = r->schur_class_ms[c] = 0.0;
+        }
         for (size_t i = 0; i + 1 < ev.size(); i += 2) {
             float t = 0.0f;
             if (hipEventElapsedTime(&t, ev[i], ev[i + 1]) != hipSuccess) t = 0.0f;
             ms += t;
             fl += flops[i / 2];
+            const int c = cls[i / 2];
+            r->schur_class_launches[c] += 1;
+            r->schur_class_flops[c] += flops[i / 2];
+            r->schur_class_ms[c] += t;
             if (large[i / 2]) {
                 ++ln;
                 lms += t;
@@ -241,6 +250,9 @@ struct EpochLog {
     }
 };
 
+// classes of Schur launches (hplai_result.schur_class_*)
+constexpr int kTrailing = 0, kBulk = 1, kInBlock = 2, kBlockRow = 3;
+
 struct LU {
     int64_t n, lda, ldq;
     int b, outer, fmt, kpad, es;
@@ -290,18 +302,21 @@ struct LU {
     }
     const float *dinv(int64_t c) const { return B->inv + (c / b) * diag_inv_floats(b); }
 
-    // trailing = true: an outer step's update of blocks that have not joined a
-    // panel yet, which goes to the FP16 trailing storage when it is on;
-    // in_block = true: an update inside the outer panel (timed apart)
+    // cls: kTrailing, an outer step's update of blocks that have not joined a
+    // panel yet (to the FP16 trailing storage when it is on); kBulk, the rows
+    // below the outer block taking the earlier panels of the block; kInBlock,
+    // an update inside the outer panel; kBlockRow, a row block of the block
+    // row.  Each class is timed apart.
     int gemm(int buf, int t0, int nsl, int64_t row0, int64_t col0, int64_t m, int64_t nn,
-             hipStream_t s, bool trailing = false, bool in_block = false) const
+             hipStream_t s, int cls) const
     {
+        const bool trailing = cls == kTrailing, in_block = cls == kInBlock;
         if (m <= 0 || nn <= 0) return HPLAI_OK;
         if (trailing && trail16) {
             const double K = static_cast<double>(nsl) * b;
             timer->begin(s, 2.0 * static_cast<double>(m) * static_cast<double>(nn) * K,
                          static_cast<double>(m + nn) * K * es + 4.0 * static_cast<double>(m) * nn,
-                         schur_large_tiles(fmt, m, nn, b, num_cu), false);
+                         schur_large_tiles(fmt, m, nn, b, num_cu), false, cls);
             const int rc = launch_schur_c16(fmt, m, nn, nsl, kpad, lq(buf, row0, t0), ldq, el(buf, t0, row0),
                                             ntiles, uq(buf, col0, t0), ldq, eu(buf, t0, col0), ntiles, b,
                                             B->P + col0 * n + row0, n, pscale, num_cu, s, packed);
@@ -311,7 +326,7 @@ struct LU {
         const double K = static_cast<double>(nsl) * b;
         timer->begin(s, 2.0 * static_cast<double>(m) * static_cast<double>(nn) * K,
                      static_cast<double>(m + nn) * K * es + 8.0 * static_cast<double>(m) * nn,
-                     schur_large_tiles(fmt, m, nn, b, num_cu), in_block);
+                     schur_large_tiles(fmt, m, nn, b, num_cu), in_block, cls);
         const int rc = launch_schur(fmt, m, nn, nsl, kpad, lq(buf, row0, t0), ldq, el(buf, t0, row0),
                                     ntiles, uq(buf, col0, t0), ldq, eu(buf, t0, col0), ntiles, b,
                                     a(row0, col0), lda, num_cu, s, packed);
@@ -375,13 +390,13 @@ struct LU {
             if (mb > 0) {
                 if (t > 0) {
                     HPLAI_HIP(hipStreamWaitEvent(q, ev_u, 0));
-                    rc = gemm(buf, 0, t, rb0, c, mb, b, q);
+                    rc = gemm(buf, 0, t, rb0, c, mb, b, q, kBulk);
                 }
                 if (rc == HPLAI_OK) rc = lazy_init(rb0, c, mb, b, q);
             }
             // chain: the diagonal block and the rows of the block column inside
             // the outer block take the t earlier inner blocks at once (K = t b)
-            if (rc == HPLAI_OK && t > 0) rc = gemm(buf, 0, t, c, c, rb0 - c, b, s, false, true);
+            if (rc == HPLAI_OK && t > 0) rc = gemm(buf, 0, t, c, c, rb0 - c, b, s, kInBlock);
             if (rc == HPLAI_OK) rc = lazy_init(c, c, rb0 - c, b, s);
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_DIAG_LU, 2.0 / 3.0 * bd * bd * bd, s, [&] {
@@ -423,7 +438,7 @@ struct LU {
             const int64_t ncols = rb0 - (c + b);
             if (rc) break;
             // block row t inside the outer block, likewise from the left (K = t b)
-            if (t > 0) rc = gemm(buf, 0, t, c, c + b, b, ncols, s, false, true);
+            if (t > 0) rc = gemm(buf, 0, t, c, c + b, b, ncols, s, kInBlock);
             if (rc == HPLAI_OK) rc = lazy_init(c, c + b, b, ncols, s);
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_TRSM_R, static_cast<double>(ncols) * bd * bd, s, [&] {
@@ -456,7 +471,7 @@ struct LU {
             const int64_t c = c0 + static_cast<int64_t>(t) * b;
             const double bd = static_cast<double>(b);
             // row block t takes the earlier row blocks at once (left-looking, K = t b)
-            if (t > 0) rc = gemm(buf, 0, t, c, col0, b, ncols, s);
+            if (t > 0) rc = gemm(buf, 0, t, c, col0, b, ncols, s, kBlockRow);
             if (rc == HPLAI_OK) rc = lazy_init(c, col0, b, ncols, s);   // block row right of the outer panel
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_TRSM_R, static_cast<double>(ncols) * bd * bd, s, [&] {
@@ -505,7 +520,7 @@ struct LU {
                     // next panel's columns first, then hand them to the side stream
                     const int64_t B1 = static_cast<int64_t>(sw1) * b;
                     const bool ahead = lookahead > 1 && rest > B1;
-                    rc = gemm(buf, 0, sw, r0, r0, rest, B1, main, true);
+                    rc = gemm(buf, 0, sw, r0, r0, rest, B1, main, kTrailing);
                     if (rc) break;
                     HPLAI_HIP(hipEventRecord(ev[0], main));
                     HPLAI_HIP(hipStreamWaitEvent(side, ev[0], 0));
@@ -514,25 +529,25 @@ struct LU {
                     rc = panel(c1, sw1, buf ^ 1, side, !ahead);
                     if (rc) break;
                     if (ahead) {
-                        rc = gemm(buf, 0, sw, r0, r0 + B1, B1, rest - B1, main, true);
+                        rc = gemm(buf, 0, sw, r0, r0 + B1, B1, rest - B1, main, kTrailing);
                         if (rc) break;
                         HPLAI_HIP(hipEventRecord(ev[2], main));
                         HPLAI_HIP(hipStreamWaitEvent(side, ev[2], 0));
                         rc = block_row(c1, sw1, buf ^ 1, side);
                         if (rc) break;
                         HPLAI_HIP(hipEventRecord(ev[1], side));
-                        rc = gemm(buf, 0, sw, r0 + B1, r0 + B1, rest - B1, rest - B1, main, true);
+                        rc = gemm(buf, 0, sw, r0 + B1, r0 + B1, rest - B1, rest - B1, main, kTrailing);
                         if (rc) break;
                         HPLAI_HIP(hipStreamWaitEvent(main, ev[6], 0));   // the panel's bulk stream
                         row_done = true;
                     } else {
                         HPLAI_HIP(hipEventRecord(ev[1], side));
-                        rc = gemm(buf, 0, sw, r0, r0 + B1, rest, rest - B1, main, true);
+                        rc = gemm(buf, 0, sw, r0, r0 + B1, rest, rest - B1, main, kTrailing);
                         if (rc) break;
                     }
                     HPLAI_HIP(hipStreamWaitEvent(main, ev[1], 0));
                 } else {
-                    rc = gemm(buf, 0, sw, r0, r0, rest, rest, main, true);
+                    rc = gemm(buf, 0, sw, r0, r0, rest, rest, main, kTrailing);
                     if (rc == HPLAI_OK) rc = panel(c1, sw1, buf ^ 1, main);
                     if (rc) break;
                 }

@@ -51,7 +51,10 @@ class Result(ctypes.Structure):
                 ("schur_launches", _i64), ("schur_flops", _dbl), ("schur_ms", _dbl),
                 ("schur_large_launches", _i64), ("schur_large_flops", _dbl), ("schur_large_ms", _dbl),
                 ("schur_large_bytes", _dbl), ("schur_inner_launches", _i64), ("schur_inner_flops", _dbl),
-                ("schur_inner_ms", _dbl), ("phase_seconds", _dbl * 21), ("phase_ops", _dbl * 21)]
+                ("schur_inner_ms", _dbl), ("schur_class_launches", _i64 * 4), ("schur_class_flops", _dbl * 4),
+                ("schur_class_ms", _dbl * 4), ("phase_seconds", _dbl * 21), ("phase_ops", _dbl * 21)]
+
+SCHUR_CLASSES = ["trailing", "rows_below", "in_block", "block_row"]
 
 PHASES = ["DIAG_BCAST", "LCOL_BCAST", "RROW_BCAST", "TEST", "WAIT", "DIAG_LU", "TRSM_L", "TRSM_R", "CONV_L",
           "CONV_R", "GEMM_UPDATE", "GEMM_PROGRESS", "LAZY_INIT", "WRITE_BACK", "IR_GEMV", "IR_GEMV_COMM",
@@ -293,6 +296,8 @@ class Context:
                     schur_large_ms=r.schur_large_ms, schur_large_bytes=r.schur_large_bytes,
                     schur_inner_launches=r.schur_inner_launches, schur_inner_flops=r.schur_inner_flops,
                     schur_inner_ms=r.schur_inner_ms,
+                    schur_classes={name: (r.schur_class_launches[i], r.schur_class_flops[i], r.schur_class_ms[i])
+                                   for i, name in enumerate(SCHUR_CLASSES)},
                     phases={PHASES[i]: (r.phase_seconds[i], r.phase_ops[i]) for i in range(21)})
 
     def solution_x(self, n):
@@ -325,7 +330,7 @@ def exported_symbols():
 # ---------------------------------------------------------------------------
 # batched solver
 # ---------------------------------------------------------------------------
-BATCH_MODES = {"persistent": 0, "graph": 1, "per_system": 2}
+BATCH_MODES = {"persistent": 0, "graph": 1, "per_system": 2, "per_system_4streams": 3, "grid_per_system": 4}
 
 
 def batch_spec(count=4096, seed=2024, nmin=32, nmax=1024, rhs_max=4):

@@ -7,6 +7,8 @@
                                     [--first N]
     python -m hplai.profsum hbm --fetch FETCH_CSV --write WRITE_CSV
                                 --copy-bytes BYTES --kernel SUBSTRING [--kernel ...]
+    python -m hplai.profsum counters COUNTER_CSV [COUNTER_CSV ...]
+                                --kernel SUBSTRING [--kernel ...] [--simds 1024]
 
 `stats` reads the `*_kernel_stats.csv` written by `rocprofv3 --kernel-trace
 --stats --output-format csv` and prints one line per kernel (calls, total and
@@ -32,6 +34,15 @@ one's HBM bytes by its summed dispatch durations from the FETCH_SIZE run
 (counter collection runs the dispatches one at a time, so these are
 standalone durations): launches, HBM bytes per launch, mean duration and
 the achieved HBM rate in GB/s.
+
+`counters` reads any `*_counter_collection.csv` files (one rocprofv3 --pmc
+pass each) and prints, per kernel, every counter summed over the kernel's
+dispatches (each dispatch's value summed over its instances, GRBM_GUI_ACTIVE
+taken as the dispatch's largest instance) with the mean dispatch duration,
+then the ratios they support: MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES /
+(GRBM_GUI_ACTIVE * SIMDs); F8 MFMA rate = 512 SQ_INSTS_VALU_MFMA_MOPS_F8 /
+duration; LDS bank-conflict cycles per LDS-active cycle; L2 hit rate
+TCC_HIT / (TCC_HIT + TCC_MISS).
 """
 import argparse
 import csv
@@ -138,6 +149,58 @@ def hbm(fetch, write, kernels, copy_bytes, copy_kernel="MulFunctor"):
                                                   (fbytes + wbytes) / ns if ns > 0 else 0.0))
 
 
+def _counter_table(paths):
+    """{(dispatch_id, kernel_name): {counter: value}} and durations (ns)."""
+    vals = defaultdict(lambda: defaultdict(float))
+    dur = {}
+    for path in paths:
+        for r in _rows(path):
+            key = (int(r["Dispatch_Id"]), r["Kernel_Name"], path)
+            c, v = r["Counter_Name"], float(r["Counter_Value"])
+            if c == "GRBM_GUI_ACTIVE":
+                vals[key][c] = max(vals[key][c], v)
+            else:
+                vals[key][c] += v
+            dur[key] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    return vals, dur
+
+
+def counters(paths, kernels, simds):
+    vals, dur = _counter_table(paths)
+    for k in kernels:
+        keys = [key for key in vals if k in key[1]]
+        if not keys:
+            raise SystemExit("kernel %r not found" % k)
+        tot = defaultdict(float)
+        runs = defaultdict(set)
+        for key in keys:
+            for c, v in vals[key].items():
+                tot[c] += v
+                runs[c].add(key[2])
+        # dispatches per pass (every pass runs the same command)
+        per_pass = len(keys) / max(1, len({key[2] for key in keys}))
+        ns = sum(dur[key] for key in keys) / max(1, len({key[2] for key in keys}))
+        print("kernel %s" % k)
+        print("dispatches_per_pass %d" % per_pass)
+        print("mean_duration_us %.3f" % (ns / per_pass * 1e-3))
+        for c in sorted(tot):
+            print("%-28s %.6g per dispatch" % (c, tot[c] / len(runs[c]) / per_pass))
+        per = {c: tot[c] / len(runs[c]) for c in tot}   # one pass's sum over the kernel's dispatches
+        if "SQ_VALU_MFMA_BUSY_CYCLES" in per and "GRBM_GUI_ACTIVE" in per:
+            print("mfma_busy_pct %.1f" % (100.0 * per["SQ_VALU_MFMA_BUSY_CYCLES"] /
+                                          (per["GRBM_GUI_ACTIVE"] * simds)))
+        if "SQ_INSTS_VALU_MFMA_MOPS_F8" in per and ns > 0:
+            print("f8_mfma_tflops %.6g" % (512.0 * per["SQ_INSTS_VALU_MFMA_MOPS_F8"] / ns * 1e-3))
+        if "SQ_INSTS_VALU_MFMA_MOPS_F16" in per and ns > 0:
+            print("f16_mfma_tflops %.6g" % (512.0 * per["SQ_INSTS_VALU_MFMA_MOPS_F16"] / ns * 1e-3))
+        if "SQ_LDS_BANK_CONFLICT" in per and per.get("SQ_LDS_IDX_ACTIVE", 0) > 0:
+            print("lds_bank_conflict_per_active_cycle %.4f" % (per["SQ_LDS_BANK_CONFLICT"] /
+                                                              per["SQ_LDS_IDX_ACTIVE"]))
+        if "TCC_HIT" in per and "TCC_MISS" in per:
+            print("l2_hit_pct %.1f" % (100.0 * per["TCC_HIT"] / (per["TCC_HIT"] + per["TCC_MISS"])))
+        print()
+
+
 def launches(path, kernel, first):
     rows = [r for r in _rows(path) if kernel in r["Kernel_Name"]]
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
@@ -175,11 +238,17 @@ def main(argv=None):
     h.add_argument("--kernel", action="append", required=True)
     h.add_argument("--copy-bytes", type=int, required=True)
     h.add_argument("--copy-kernel", default="MulFunctor")
+    c = sub.add_parser("counters")
+    c.add_argument("csv", nargs="+")
+    c.add_argument("--kernel", action="append", required=True)
+    c.add_argument("--simds", type=int, default=1024, help="SIMDs on the device (MI355X: 256 CUs x 4)")
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         stats(a.csv, a.top)
     elif a.cmd == "launches":
         launches(a.csv, a.kernel, a.first)
+    elif a.cmd == "counters":
+        counters(a.csv, a.kernel, a.simds)
     elif a.cmd == "hbm":
         hbm(a.fetch, a.write, a.kernel, a.copy_bytes, a.copy_kernel)
     else:

@@ -209,7 +209,9 @@ int hplai_batch_get_nb(hplai_batch b, int32_t nb[4]);
  * and keeps the fastest; nb_out (may be NULL) receives the table */
 int hplai_batch_autotune(hplai_batch b, int maxit, int32_t nb_out[4]);
 /* mode 0: one persistent launch; 1: the same launch captured once in a
- * hipGraph and replayed; 2: one launch per system (baseline).  Synchronous. */
+ * hipGraph and replayed; baselines: 2: one launch per system on one stream;
+ * 3: one launch per system round-robin over four streams; 4: one launch of
+ * one workgroup per system in input order.  Synchronous. */
 int hplai_batch_run(hplai_batch b, int mode, int maxit, hplai_batch_stats *st);
 /* per system: harness[4 s + r], steps[4 s + r] (r < 4; -1 / 0 past nrhs),
  * info[s], time_us[s]; any pointer may be NULL (host arrays) */
@@ -306,6 +308,13 @@ typedef struct hplai_result {
     int64_t schur_inner_launches;
     double  schur_inner_flops;
     double  schur_inner_ms;
+    /* with profile: every Schur launch by class -- [0] the outer steps' trailing
+       updates (K = outer b), [1] the rows below the outer block taking the
+       earlier panels of the block (K = t b), [2] the updates inside the outer
+       panel, [3] the row blocks of the block row (K = t b) */
+    int64_t schur_class_launches[4];
+    double  schur_class_flops[4];
+    double  schur_class_ms[4];
     /* with profile: per-phase device time (seconds, summed kernel durations
        from HIP events; phases on the two streams overlap) and operation
        counts as the reference's Timer accumulates them; TOTAL is the LU + IR

@@ -72,3 +72,41 @@ def test_hbm_rate_uses_calibrated_bytes_and_fetch_run_durations(tmp_path, capsys
     assert abs(per / (2.5 * gib) - 1) < 1e-5
     assert abs(mean_us - 2.0) < 1e-9
     assert abs(rate / (2 * 2.5 * gib / 4000) - 1) < 1e-5
+
+
+def test_counters_ratios(tmp_path, capsys):
+    """`counters`: per-kernel sums over dispatches (instances summed,
+    GRBM_GUI_ACTIVE as the largest instance), MFMA busy over GUI-active
+    cycles times SIMDs, F8 rate from the MOPS count, L2 hit rate."""
+    fields = ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value", "Start_Timestamp", "End_Timestamp"]
+    k = "schur_dma_kernel<2, 2, true>"
+    p1 = tmp_path / "a.csv"
+    with open(p1, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=fields)
+        w.writeheader()
+        for d in (1, 2):
+            for inst in range(2):   # two instances: summed, GUI active as the max
+                w.writerow(dict(Dispatch_Id=d, Kernel_Name=k, Counter_Name="SQ_VALU_MFMA_BUSY_CYCLES",
+                                Counter_Value=512 * 500, Start_Timestamp=0, End_Timestamp=1000))
+                w.writerow(dict(Dispatch_Id=d, Kernel_Name=k, Counter_Name="GRBM_GUI_ACTIVE",
+                                Counter_Value=1000 - inst, Start_Timestamp=0, End_Timestamp=1000))
+                w.writerow(dict(Dispatch_Id=d, Kernel_Name=k, Counter_Name="SQ_INSTS_VALU_MFMA_MOPS_F8",
+                                Counter_Value=1000, Start_Timestamp=0, End_Timestamp=1000))
+    p2 = tmp_path / "b.csv"
+    with open(p2, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=fields)
+        w.writeheader()
+        for d in (1, 2):
+            w.writerow(dict(Dispatch_Id=d, Kernel_Name=k, Counter_Name="TCC_HIT", Counter_Value=3,
+                            Start_Timestamp=0, End_Timestamp=1000))
+            w.writerow(dict(Dispatch_Id=d, Kernel_Name=k, Counter_Name="TCC_MISS", Counter_Value=1,
+                            Start_Timestamp=0, End_Timestamp=1000))
+    profsum.main(["counters", str(p1), str(p2), "--kernel", "schur_dma", "--simds", "1024"])
+    out = capsys.readouterr().out
+    vals = dict(ln.split()[:2] for ln in out.splitlines() if ln and not ln.startswith("kernel"))
+    assert vals["dispatches_per_pass"] == "2"
+    # 2 x 512 x 500 busy cycles per dispatch over 1000 x 1024 SIMD-cycles
+    assert abs(float(vals["mfma_busy_pct"]) - 50.0) < 1e-9
+    # 2 instances x 1000 MOPS x 512 flops per dispatch over 1000 ns
+    assert abs(float(vals["f8_mfma_tflops"]) - 2 * 1000 * 512 / 1000 * 1e-3) < 1e-9
+    assert abs(float(vals["l2_hit_pct"]) - 75.0) < 1e-9
