@@ -75,6 +75,12 @@ int hplai_create(int device, hplai_ctx *out)
         hplai_destroy(c);
         return hip_fail(e, "hplai_create streams/events");
     }
+    // the device address of this device's no-allocation scratch array
+    const int rc = noalloc_setup(device, &c->scratch);
+    if (rc != HPLAI_OK) {
+        hplai_destroy(c);
+        return rc;
+    }
     *out = c;
     return HPLAI_OK;
 }

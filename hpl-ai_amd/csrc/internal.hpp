@@ -16,6 +16,8 @@ struct hplai_ctx_s {
     float *A = nullptr;
     int64_t n = 0, lda = 0;
     double *x = nullptr;
+    // device address of this device's scratch array (hplai_otf_gemv_f64_noalloc)
+    void *scratch = nullptr;
 };
 
 namespace hplai {
@@ -106,6 +108,8 @@ int launch_gemmschur_f16(int64_t n, int64_t m, int64_t k, const uint16_t *a, int
 int launch_otf_gemv(int64_t n, uint64_t seed, const double *d, double alpha, const double *x,
                     double *y, double *work, hipStream_t s);
 int64_t otf_gemv_work_doubles(int64_t n);
+// address of the no-allocation scratch array on `device` (current), looked up once per device
+int noalloc_setup(int device, void **out);
 // triangular solves with the FP32 factors of LU, FP64 vectors; nrhs columns
 // of x with stride ldx
 bool tri_inverse_ok(int b);

@@ -86,6 +86,7 @@ _SIGS = [
     ("hplai_schur_update_f16c", _int, [_vp, _int, _i64, _i64, _int, _int, _vp, _i64, _vp, _i64, _vp, _i64,
                                        _vp, _i64, _int, _vp, _i64, ctypes.c_float, _vp]),
     ("hplai_otf_gemv_f64", _int, [_vp, _i64, _u64, _vp, _dbl, _vp, _vp, _vp]),
+    ("hplai_otf_gemv_f64_noalloc", _int, [_vp, _i64, _u64, _vp, _dbl, _vp, _vp, _vp]),
     ("hplai_higham_coeffs", _int, [_i64, _dbl, _dbl, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]),
     ("hplai_gen_higham_f32", _int, [_vp, _i64, _dbl, _dbl, _i64, _i64, _i64, _i64, _vp, _i64, _vp]),
     ("hplai_higham_gemv_f64", _int, [_vp, _i64, _dbl, _dbl, _dbl, _vp, _vp, _vp]),
@@ -248,6 +249,12 @@ class Context:
     def otf_gemv(self, n, seed, d, alpha, x, y, stream=None):
         check(lib().hplai_otf_gemv_f64(self.handle, n, seed, ptr(d), alpha, ptr(x), ptr(y),
                                        stream_ptr(stream)))
+
+    def otf_gemv_noalloc(self, n, seed, d, alpha, x, y, stream=None):
+        """otf_gemv without a device allocation (partial sums in the library's
+        per-device 32 MiB scratch array; n <= 92160)."""
+        check(lib().hplai_otf_gemv_f64_noalloc(self.handle, n, seed, ptr(d), alpha, ptr(x), ptr(y),
+                                               stream_ptr(stream)))
 
     def trsv_lower(self, n, bs, LU, lda, x, stream=None):
         check(lib().hplai_trsv_lower_f32(self.handle, n, bs, ptr(LU), lda, ptr(x), stream_ptr(stream)))

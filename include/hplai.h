@@ -149,6 +149,15 @@ int hplai_higham_gemv_f64(hplai_ctx ctx, int64_t n, double alpha, double beta, d
  * from (n, seed) and diagonal d (device, FP64). */
 int hplai_otf_gemv_f64(hplai_ctx ctx, int64_t n, uint64_t seed, const double *d, double alpha,
                        const double *x, double *y, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): hplai_otf_gemv_f64 with the same kernels and results,
+ * its FP64 partial sums kept in a 32 MiB device array that the library holds
+ * per device until the process ends, so the call allocates nothing.
+ * n * ceil(n / 2048) * 8 bytes must fit in it (n <= 92160), otherwise
+ * HPLAI_EINVAL.  Calls that overlap on one device share the array and race. */
+int hplai_otf_gemv_f64_noalloc(hplai_ctx ctx, int64_t n, uint64_t seed, const double *d, double alpha,
+                               const double *x, double *y, void *stream);
 /* Replaces panel_trsvL / tttrsvL (panel_trsv.hpp:115-124, 310-535):
  * x := L^-1 x, L the unit lower triangle of LU (FP32), x FP64.  bs is the
  * blocking (<= 1024; use the LU block size). */
