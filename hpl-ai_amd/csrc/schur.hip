@@ -53,6 +53,24 @@ __device__ __forceinline__ int xcd_remap(int w, int nwg)
     return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
 
+// C is read once and written once per update, so its accesses carry the
+// non-temporal hint: they stream past the caches instead of evicting the
+// operand panels that the other workgroups of the XCD are still reading
+// (e4m3, 32768^2 x 2048: 3.56 -> 2.85 ms; x 8192: 8.29 -> 8.14 ms).
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 load_c_nt(const float *p)
+{
+    const floatx4_t v = __builtin_nontemporal_load(reinterpret_cast<const floatx4_t *>(p));
+    return float4{v[0], v[1], v[2], v[3]};
+}
+
+__device__ __forceinline__ void store_c_nt(float *p, const float4 &c)
+{
+    const floatx4_t v = {c.x, c.y, c.z, c.w};
+    __builtin_nontemporal_store(v, reinterpret_cast<floatx4_t *>(p));
+}
+
 // C -= acc for JB accumulator blocks side by side (columns col0 + 32 j ..),
 // rows row0 .. row0 + 31.  acc[j][4 q + r] is row 8 q + 4 lgrp + r, column
 // lrow of block j.  With VEC_C every lane loads its 4 JB float4 first
@@ -72,7 +90,7 @@ __device__ __forceinline__ void epilogue_block_row(float *__restrict__ C, int64_
             for (int q = 0; q < 4; ++q) {
                 const int64_t gc = col0 + j * 32 + lrow, gr = row0 + 8 * q + 4 * lgrp;
                 const bool full = gc < N && gr + 3 < M;
-                cv[j][q] = *reinterpret_cast<const float4 *>(full ? C + gc * ldc + gr : C);
+                cv[j][q] = load_c_nt(full ? C + gc * ldc + gr : C);
             }
 #pragma unroll
         for (int j = 0; j < JB; ++j)
@@ -85,7 +103,7 @@ __device__ __forceinline__ void epilogue_block_row(float *__restrict__ C, int64_
                     c4.y -= acc[j][4 * q + 1];
                     c4.z -= acc[j][4 * q + 2];
                     c4.w -= acc[j][4 * q + 3];
-                    *reinterpret_cast<float4 *>(C + gc * ldc + gr) = c4;
+                    store_c_nt(C + gc * ldc + gr, c4);
                 } else if (gc < N) {
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
@@ -136,7 +154,7 @@ __device__ __forceinline__ void epilogue_block_row_lds(float *__restrict__ C, in
         float4 cv[8];
 #pragma unroll
         for (int p = 0; p < 8; ++p)
-            cv[p] = *reinterpret_cast<const float4 *>(Cb + static_cast<int64_t>((8 * h + p) * 8 + cl) * ldc);
+            cv[p] = load_c_nt(Cb + static_cast<int64_t>((8 * h + p) * 8 + cl) * ldc);
 #pragma unroll
         for (int p = 0; p < 8; ++p) {
             const float4 a = *reinterpret_cast<const float4 *>(T + ((8 * h + p) * 8 + cl) * kEpiTS + rq);
@@ -145,7 +163,7 @@ __device__ __forceinline__ void epilogue_block_row_lds(float *__restrict__ C, in
             c4.y -= a.y;
             c4.z -= a.z;
             c4.w -= a.w;
-            *reinterpret_cast<float4 *>(Cb + static_cast<int64_t>((8 * h + p) * 8 + cl) * ldc) = c4;
+            store_c_nt(Cb + static_cast<int64_t>((8 * h + p) * 8 + cl) * ldc, c4);
         }
     }
 }

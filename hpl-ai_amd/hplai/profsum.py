@@ -38,9 +38,10 @@ the achieved HBM rate in GB/s.
 `counters` reads any `*_counter_collection.csv` files (one rocprofv3 --pmc
 pass each) and prints, per kernel, every counter summed over the kernel's
 dispatches (each dispatch's value summed over its instances, GRBM_GUI_ACTIVE
-taken as the dispatch's largest instance) with the mean dispatch duration,
-then the ratios they support: MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES /
-(GRBM_GUI_ACTIVE * SIMDs); F8 MFMA rate = 512 SQ_INSTS_VALU_MFMA_MOPS_F8 /
+summed over the XCDs, so GRBM_GUI_ACTIVE / XCDs is the dispatch's length in
+shader clocks) with the mean dispatch duration, then the ratios they support:
+the shader clock = GRBM_GUI_ACTIVE / XCDs / duration; MFMA busy =
+SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / XCDs * SIMDs); F8 MFMA rate = 512 SQ_INSTS_VALU_MFMA_MOPS_F8 /
 duration; LDS bank-conflict cycles per LDS-active cycle; L2 hit rate
 TCC_HIT / (TCC_HIT + TCC_MISS).
 """
@@ -157,15 +158,12 @@ def _counter_table(paths):
         for r in _rows(path):
             key = (int(r["Dispatch_Id"]), r["Kernel_Name"], path)
             c, v = r["Counter_Name"], float(r["Counter_Value"])
-            if c == "GRBM_GUI_ACTIVE":
-                vals[key][c] = max(vals[key][c], v)
-            else:
-                vals[key][c] += v
+            vals[key][c] += v
             dur[key] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
     return vals, dur
 
 
-def counters(paths, kernels, simds):
+def counters(paths, kernels, simds, xcds=8):
     vals, dur = _counter_table(paths)
     for k in kernels:
         keys = [key for key in vals if k in key[1]]
@@ -186,9 +184,12 @@ def counters(paths, kernels, simds):
         for c in sorted(tot):
             print("%-28s %.6g per dispatch" % (c, tot[c] / len(runs[c]) / per_pass))
         per = {c: tot[c] / len(runs[c]) for c in tot}   # one pass's sum over the kernel's dispatches
+        if "GRBM_GUI_ACTIVE" in per and ns > 0:
+            # GRBM_GUI_ACTIVE comes summed over the XCDs: cycles per XCD over the duration is the clock
+            print("sclk_mhz %.0f" % (per["GRBM_GUI_ACTIVE"] / xcds / ns * 1e3))
         if "SQ_VALU_MFMA_BUSY_CYCLES" in per and "GRBM_GUI_ACTIVE" in per:
             print("mfma_busy_pct %.1f" % (100.0 * per["SQ_VALU_MFMA_BUSY_CYCLES"] /
-                                          (per["GRBM_GUI_ACTIVE"] * simds)))
+                                          (per["GRBM_GUI_ACTIVE"] / xcds * simds)))
         if "SQ_INSTS_VALU_MFMA_MOPS_F8" in per and ns > 0:
             print("f8_mfma_tflops %.6g" % (512.0 * per["SQ_INSTS_VALU_MFMA_MOPS_F8"] / ns * 1e-3))
         if "SQ_INSTS_VALU_MFMA_MOPS_F16" in per and ns > 0:
@@ -196,8 +197,9 @@ def counters(paths, kernels, simds):
         if "SQ_LDS_BANK_CONFLICT" in per and per.get("SQ_LDS_IDX_ACTIVE", 0) > 0:
             print("lds_bank_conflict_per_active_cycle %.4f" % (per["SQ_LDS_BANK_CONFLICT"] /
                                                               per["SQ_LDS_IDX_ACTIVE"]))
-        if "TCC_HIT" in per and "TCC_MISS" in per:
-            print("l2_hit_pct %.1f" % (100.0 * per["TCC_HIT"] / (per["TCC_HIT"] + per["TCC_MISS"])))
+        for hit, miss in (("TCC_HIT", "TCC_MISS"), ("TCC_HIT_sum", "TCC_MISS_sum")):
+            if hit in per and miss in per:
+                print("l2_hit_pct %.1f" % (100.0 * per[hit] / (per[hit] + per[miss])))
         print()
 
 
@@ -242,13 +244,14 @@ def main(argv=None):
     c.add_argument("csv", nargs="+")
     c.add_argument("--kernel", action="append", required=True)
     c.add_argument("--simds", type=int, default=1024, help="SIMDs on the device (MI355X: 256 CUs x 4)")
+    c.add_argument("--xcds", type=int, default=8, help="XCDs on the device (GRBM_GUI_ACTIVE is summed over them)")
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         stats(a.csv, a.top)
     elif a.cmd == "launches":
         launches(a.csv, a.kernel, a.first)
     elif a.cmd == "counters":
-        counters(a.csv, a.kernel, a.simds)
+        counters(a.csv, a.kernel, a.simds, a.xcds)
     elif a.cmd == "hbm":
         hbm(a.fetch, a.write, a.kernel, a.copy_bytes, a.copy_kernel)
     else:

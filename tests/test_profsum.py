@@ -75,9 +75,10 @@ def test_hbm_rate_uses_calibrated_bytes_and_fetch_run_durations(tmp_path, capsys
 
 
 def test_counters_ratios(tmp_path, capsys):
-    """`counters`: per-kernel sums over dispatches (instances summed,
-    GRBM_GUI_ACTIVE as the largest instance), MFMA busy over GUI-active
-    cycles times SIMDs, F8 rate from the MOPS count, L2 hit rate."""
+    """`counters`: per-kernel sums over dispatches (instances summed, also
+    GRBM_GUI_ACTIVE, which counts once per XCD), the clock from GUI-active
+    cycles per XCD, MFMA busy over those cycles times SIMDs, F8 rate from the
+    MOPS count, L2 hit rate."""
     fields = ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value", "Start_Timestamp", "End_Timestamp"]
     k = "schur_dma_kernel<2, 2, true>"
     p1 = tmp_path / "a.csv"
@@ -85,11 +86,11 @@ def test_counters_ratios(tmp_path, capsys):
         w = csv.DictWriter(f, fieldnames=fields)
         w.writeheader()
         for d in (1, 2):
-            for inst in range(2):   # two instances: summed, GUI active as the max
+            for inst in range(2):   # two instances (XCDs): summed
                 w.writerow(dict(Dispatch_Id=d, Kernel_Name=k, Counter_Name="SQ_VALU_MFMA_BUSY_CYCLES",
                                 Counter_Value=512 * 500, Start_Timestamp=0, End_Timestamp=1000))
                 w.writerow(dict(Dispatch_Id=d, Kernel_Name=k, Counter_Name="GRBM_GUI_ACTIVE",
-                                Counter_Value=1000 - inst, Start_Timestamp=0, End_Timestamp=1000))
+                                Counter_Value=1000, Start_Timestamp=0, End_Timestamp=1000))
                 w.writerow(dict(Dispatch_Id=d, Kernel_Name=k, Counter_Name="SQ_INSTS_VALU_MFMA_MOPS_F8",
                                 Counter_Value=1000, Start_Timestamp=0, End_Timestamp=1000))
     p2 = tmp_path / "b.csv"
@@ -101,10 +102,12 @@ def test_counters_ratios(tmp_path, capsys):
                             Start_Timestamp=0, End_Timestamp=1000))
             w.writerow(dict(Dispatch_Id=d, Kernel_Name=k, Counter_Name="TCC_MISS", Counter_Value=1,
                             Start_Timestamp=0, End_Timestamp=1000))
-    profsum.main(["counters", str(p1), str(p2), "--kernel", "schur_dma", "--simds", "1024"])
+    profsum.main(["counters", str(p1), str(p2), "--kernel", "schur_dma", "--simds", "1024", "--xcds", "2"])
     out = capsys.readouterr().out
     vals = dict(ln.split()[:2] for ln in out.splitlines() if ln and not ln.startswith("kernel"))
     assert vals["dispatches_per_pass"] == "2"
+    # 1000 cycles per XCD in 1000 ns
+    assert abs(float(vals["sclk_mhz"]) - 1000.0) < 1e-9
     # 2 x 512 x 500 busy cycles per dispatch over 1000 x 1024 SIMD-cycles
     assert abs(float(vals["mfma_busy_pct"]) - 50.0) < 1e-9
     # 2 instances x 1000 MOPS x 512 flops per dispatch over 1000 ns
