@@ -95,6 +95,8 @@ _SIGS = [
     ("hplai_norm_inf_f64", _int, [_vp, _i64, _vp, _vp, _vp]),
     ("hplai_factor_f32", _int, [_vp, _i64, _int, _int, _int, _int, _vp, _i64]),
     ("hplai_lu_solve_f64", _int, [_vp, _int, _i64, _int, _int, _vp, _i64, _vp, _i64, _vp]),
+    ("hplai_gemm_f64", _int, [_vp, _int, _int, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, _dbl, _vp, _i64,
+                              _vp]),
     ("hplai_batch_create", _int, [_vp, _int, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
     ("hplai_batch_destroy", _int, [_vp]),
     ("hplai_batch_set_nb", _int, [_vp, _vp]),
@@ -270,6 +272,11 @@ class Context:
         """LU of the n x n FP32 column-major device matrix A in place."""
         code = FMTS[fmt] if isinstance(fmt, str) else fmt
         check(lib().hplai_factor_f32(self.handle, n, b, code, outer, int(lookahead), ptr(A), lda))
+
+    def gemm_f64(self, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream=None):
+        """C := alpha op(A) op(B) + beta C in FP64 (column-major, op = transpose when trans*)."""
+        check(lib().hplai_gemm_f64(self.handle, 1 if transa else 0, 1 if transb else 0, m, n, k, alpha, ptr(A),
+                                   lda, ptr(B), ldb, beta, ptr(C), ldc, stream_ptr(stream)))
 
     def lu_solve(self, n, nrhs, bs, LU, lda, X, ldx, trans=False, stream=None):
         """X := (LU)^-1 X, or (LU)^-T X with trans; X FP64 with nrhs columns."""

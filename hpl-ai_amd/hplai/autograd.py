@@ -12,9 +12,11 @@ the transposed system A^T Lambda = W (`hplai_lu_solve_f64` with trans = 1,
 i.e. U^-T then L^-T), refined in FP64 against A^T; then dA = -Lambda X^T (one
 FP64 GEMM) and dB = Lambda.  The factorization is not repeated.
 
-The residual products A X and A^T Lambda and the gradient GEMM are plain
-FP64 matrix products done by torch (rocBLAS); the factorization and the
-triangular solves are the library's HIP kernels.
+The factorization, the triangular solves and the FP64 products (the
+residuals B - A X and W - A^T Lambda and the gradient -Lambda X^T, on the
+FP64 matrix cores: `hplai_gemm_f64`) are the library's HIP kernels; torch
+does the elementwise work around them (transposed copies, x += d, the norms
+of the harness).
 """
 import torch
 
@@ -28,6 +30,30 @@ def harness(A_norm, R, X, B, n):
     return R.abs().amax(0) / (A_norm * X.abs().amax(0) + B.abs().amax(0)) / (n * DBL_EPS / 2)
 
 
+def residual(hctx, A, X, B, trans=False):
+    """R = B - op(A) X in FP64 with the library's GEMM (op = transpose when
+    `trans`).  A (n, n), X and B (n, k): contiguous float64 CUDA tensors.
+
+    A row-major (r, c) tensor is the column-major c x r matrix of its
+    transpose, so R^T = B^T - X^T op(A)^T is one column-major GEMM with
+    m = k, n = n, K = n: op(A)^T is A's own column-major view (A^T) for
+    trans = False and its transpose for trans = True."""
+    n, k = X.shape
+    R = B.clone()
+    hctx.gemm_f64(False, trans, k, n, n, -1.0, X, k, A, n, 1.0, R, k)
+    return R
+
+
+def outer_grad(hctx, Lam, X):
+    """dA = -Lam X^T (n x n) with the library's GEMM: column-major
+    dA^T = -X Lam^T, i.e. op(A) = X (the transpose of X's column-major view)
+    and op(B) = Lam's column-major view, K = k."""
+    n, k = X.shape
+    dA = torch.empty((n, n), dtype=torch.float64, device=X.device)
+    hctx.gemm_f64(True, False, n, n, k, -1.0, X, k, Lam, k, 0.0, dA, n)
+    return dA
+
+
 def refine(hctx, A, F, bs, B, trans=False, tol=16.0, maxit=50, min_steps=0):
     """Solve op(A) X = B (op = transpose when `trans`) with the FP32 factors F
     (column-major, leading dimension n) and FP64 refinement.
@@ -35,19 +61,20 @@ def refine(hctx, A, F, bs, B, trans=False, tol=16.0, maxit=50, min_steps=0):
     A: (n, n) float64 CUDA tensor; B: (n, k) float64.  Returns (X, steps,
     harness per column)."""
     n, k = B.shape
-    Aop = A.t() if trans else A
-    norma = Aop.abs().sum(1).amax()
+    A = A.contiguous()
+    B = B.contiguous()
+    norma = (A.abs().sum(0) if trans else A.abs().sum(1)).amax()
     Xc = B.t().contiguous()                 # column j of X at Xc[j]: leading dimension n
     hctx.lu_solve(n, k, bs, F, n, Xc, n, trans=trans)
     h = None
     steps = 0
     for it in range(maxit):
-        X = Xc.t()
-        R = B - Aop @ X
+        X = Xc.t().contiguous()
+        R = residual(hctx, A, X, B, trans)
         h = harness(norma, R, X, B, n)
         steps = it + 1
         if it >= min_steps and bool((h < tol).all()):
-            return X.contiguous(), steps, h
+            return X, steps, h
         Rc = R.t().contiguous()
         hctx.lu_solve(n, k, bs, F, n, Rc, n, trans=trans)
         Xc += Rc
@@ -85,7 +112,7 @@ class LUSolve(torch.autograd.Function):
                                opts["maxit"], opts["min_steps"])
         opts["backward_steps"] = steps
         opts["backward_harness"] = float(h.max())
-        dA = -(Lam @ X.t()) if fctx.needs_input_grad[0] else None
+        dA = outer_grad(opts["ctx"], Lam, X) if fctx.needs_input_grad[0] else None
         dB = Lam if fctx.needs_input_grad[1] else None
         return dA, dB, None
 

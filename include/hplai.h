@@ -188,6 +188,16 @@ int hplai_factor_f32(hplai_ctx ctx, int64_t n, int b, int fmt, int outer, int lo
 int hplai_lu_solve_f64(hplai_ctx ctx, int trans, int64_t n, int nrhs, int bs, const float *LU,
                        int64_t lda, double *X, int64_t ldx, void *stream);
 
+/* No reference counterpart (it serves the differentiable form): FP64 GEMM on
+ * the FP64 matrix cores, C := alpha op(A) op(B) + beta C, column-major, op(X)
+ * = X^T when trans* != 0, C m x n, op(A) m x k; beta = 0 does not read C.
+ * Used for the refinement residuals R = B - op(A) X and the gradient
+ * dA = -Lambda X^T of hplai.autograd; results do not depend on scheduling
+ * (a split K is summed in a fixed order). */
+int hplai_gemm_f64(hplai_ctx ctx, int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
+                   const double *A, int64_t lda, const double *B, int64_t ldb, double beta, double *C,
+                   int64_t ldc, void *stream);
+
 /* ---------------- batched solver (no reference counterpart) ----------------
  * Many independent HPL-AI systems (order 1..1024, 1..4 FP64 right-hand
  * sides, each built by the reference generator from its own seed: element

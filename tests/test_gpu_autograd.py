@@ -71,6 +71,42 @@ def test_factor_equals_solve_factors(ctx, fmt, n, b):
     assert np.array_equal(F.cpu().numpy().view(np.uint32), np.ascontiguousarray(lu_solve).view(np.uint32))
 
 
+@pytest.mark.parametrize("m,n,k,ta,tb,beta", [
+    (100, 70, 33, False, False, 0.0), (257, 129, 1000, True, True, 0.5),
+    (1000, 1000, 64, True, False, 0.0),          # the gradient's shape (rank-64 product)
+    (64, 8192, 8192, False, False, 1.0),         # the residuals: K split over workgroups
+    (64, 2048, 2048, False, True, 1.0), (300, 200, 0, False, False, 2.0)])
+def test_gemm_f64_vs_fp64(ctx, m, n, k, ta, tb, beta):
+    """hplai_gemm_f64 against an FP64 product on the CPU: |C - ref| <=
+    (k + 2) 2^-52 (|alpha| |op(A)| |op(B)| + |beta C0|), the FP64 summation
+    bound (the library adds in a different order); a second call gives the same
+    bits (a split K is summed in a fixed order)."""
+    import torch
+    rng = np.random.default_rng(m + n + k)
+    alpha = -1.0
+    Ash = (k, m) if ta else (m, k)
+    Bsh = (n, k) if tb else (k, n)
+    A = rng.standard_normal(Ash)
+    B = rng.standard_normal(Bsh)
+    C0 = rng.standard_normal((m, n))
+    opA = A.T if ta else A
+    opB = B.T if tb else B
+    ref = alpha * (opA @ opB) + beta * C0
+    bound = (k + 2) * 2.0 ** -52 * (abs(alpha) * (np.abs(opA) @ np.abs(opB)) + abs(beta) * np.abs(C0)) + 1e-300
+    # column-major device copies: a column-major r x c matrix is the row-major (c, r) tensor of its transpose
+    dA = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+    dB = torch.from_numpy(np.ascontiguousarray(B.T)).cuda()
+    outs = []
+    for _ in range(2):
+        dC = torch.from_numpy(np.ascontiguousarray(C0.T)).cuda()
+        ctx.gemm_f64(ta, tb, m, n, k, alpha, dA, Ash[0], dB, Bsh[0], beta, dC, m)
+        torch.cuda.synchronize()
+        outs.append(dC.cpu().numpy().T)
+    assert np.array_equal(outs[0].view(np.uint64), outs[1].view(np.uint64))
+    ratio = np.max(np.abs(outs[0] - ref) / bound)
+    assert ratio < 1.0, ratio
+
+
 def test_factor_zero_pivot(ctx):
     import hplai
     import torch
