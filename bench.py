@@ -42,17 +42,23 @@ def _storage(args):
     return dict(lazy=bool(args.lazy or args.trail16), trail16=bool(args.trail16))
 
 
+def _split(args, outer):
+    """the two-part panel split, where the form's outer block allows it"""
+    return args.split if 0 < args.split < outer else 0
+
+
 def run_form(ctx, args, fmt, outer):
     res = None
     for _ in range(args.warmup):
-        ctx.solve(args.n, args.b, fmt=fmt, outer=outer, lookahead=_lookahead(args), maxit=50, **_storage(args))
+        ctx.solve(args.n, args.b, fmt=fmt, outer=outer, lookahead=_lookahead(args), maxit=50,
+                  split=_split(args, outer), **_storage(args))
     import torch
     times, results = [], []
     torch.cuda.synchronize()
     w0 = time.perf_counter()
     for _ in range(args.steps):
         r = ctx.solve(args.n, args.b, fmt=fmt, outer=outer, lookahead=_lookahead(args),
-                      maxit=50, profile=True, **_storage(args))
+                      maxit=50, profile=True, split=_split(args, outer), **_storage(args))
         if not r["converged"]:
             raise SystemExit("%s did not reach harness < 16: %s" % (fmt, r))
         times.append(r["time_total"])
@@ -73,7 +79,7 @@ def run_form(ctx, args, fmt, outer):
         classes[name] = {"launches": nl // len(results), "ms_per_solve": ms / len(results),
                          "tflops": fl / (ms * 1e-3) * 1e-12 if ms > 0 else None}
     tot["schur_classes"] = classes
-    return dict(fmt=fmt, outer=outer, time_s=t, wall_s=wall, tflops=hpl_flops(args.n) / t * 1e-12,
+    return dict(fmt=fmt, outer=outer, split=_split(args, outer), time_s=t, wall_s=wall, tflops=hpl_flops(args.n) / t * 1e-12,
                 time_lu_s=res["time_lu"], time_ir_s=res["time_ir"], time_gen_s=res["time_gen"],
                 ir_steps=res["ir_steps"], harness=res["harness"], residual=res["residual"],
                 schur_tflops=(tot["schur_flops"] / (tot["schur_ms"] * 1e-3) * 1e-12)
@@ -326,6 +332,8 @@ def main():
     ap.add_argument("--outer", type=int, default=16)
     ap.add_argument("--compare", default="f16", help="second form run beside the primary ('' = none)")
     ap.add_argument("--compare-outer", type=int, default=16)
+    ap.add_argument("--split", type=int, default=0,
+                    help="factor each outer panel in two parts after this many inner blocks (0 = one part)")
     ap.add_argument("--no-lookahead", action="store_true")
     ap.add_argument("--lazy", action="store_true", help="lazy initialisation (-d)")
     ap.add_argument("--trail16", action="store_true",
@@ -403,7 +411,7 @@ def main():
                                   ", lazy init, FP16 trailing storage" if args.trail16
                                   else (", lazy init" if args.lazy else "")),
                    "n": args.n, "b": args.b, "fmt": args.fmt, "outer": args.outer,
-                   "lookahead": _lookahead(args), **_storage(args)},
+                   "lookahead": _lookahead(args), "split": _split(args, args.outer), **_storage(args)},
         "time_to_harness_s": primary["time_s"],
         "ir_steps": primary["ir_steps"],
         "harness": primary["harness"],
@@ -420,7 +428,7 @@ def main():
         line["fp64_comparison"] = fp64
         line["speedup_vs_fp64_extrapolated"] = fp64["fp64_time_extrapolated_to_n%d_s" % args.n] / primary["time_s"]
     if compare is not None:
-        line["compare"] = {k: compare[k] for k in ("fmt", "outer", "time_s", "tflops", "ir_steps",
+        line["compare"] = {k: compare[k] for k in ("fmt", "outer", "split", "time_s", "tflops", "ir_steps",
                                                    "harness", "time_lu_s", "time_ir_s",
                                                    "schur_tflops")}
         line["compare"]["backward_error"] = backward_error(compare["harness"], args.n)

@@ -256,6 +256,11 @@ constexpr int kTrailing = 0, kBulk = 1, kInBlock = 2, kBlockRow = 3;
 struct LU {
     int64_t n, lda, ldq;
     int b, outer, fmt, kpad, es;
+    // 0, or h (0 < h < outer): the outer panel is factored in two parts; after
+    // its first h inner blocks the rest of the outer block (and of its block
+    // row) takes them in one update (K = h b), and the later inner blocks
+    // look left only as far as block h
+    int split = 0;
     bool lazy;
     bool higham = false;          // Higham's matrix instead of the LCG matrix
     bool packed = false;          // Lq / Uq in the panel-packed layout (b % 256 == 0)
@@ -381,22 +386,31 @@ struct LU {
             HPLAI_HIP(hipEventRecord(ev_fork, s));
             HPLAI_HIP(hipStreamWaitEvent(q, ev_fork, 0));
         }
+        const bool two = split > 0 && split < sw;   // the panel is factored in two parts
         for (int t = 0; t < sw && rc == HPLAI_OK; ++t) {
             const int64_t c = c0 + static_cast<int64_t>(t) * b;
             const double bd = static_cast<double>(b);
+            // inner blocks base .. t - 1 are applied per block column; with the
+            // split, at t == split the rest of the outer block takes blocks
+            // 0 .. split - 1 in one update (mid) and later blocks start there
+            const bool mid = two && t == split;
+            const int base = two && t >= split ? split : 0, nsl = t - base;
             // bulk: the rows below take all earlier panels of the block at once
             // (left-looking, K = t b) once the chain has quantized U of block
             // t - 1 (the wait refers to the record of the previous iteration)
             if (mb > 0) {
                 if (t > 0) {
                     HPLAI_HIP(hipStreamWaitEvent(q, ev_u, 0));
-                    rc = gemm(buf, 0, t, rb0, c, mb, b, q, kBulk);
+                    if (mid) rc = gemm(buf, 0, split, rb0, c, mb, rb0 - c, q, kBulk);
+                    if (rc == HPLAI_OK && nsl > 0) rc = gemm(buf, base, nsl, rb0, c, mb, b, q, kBulk);
                 }
                 if (rc == HPLAI_OK) rc = lazy_init(rb0, c, mb, b, q);
             }
             // chain: the diagonal block and the rows of the block column inside
-            // the outer block take the t earlier inner blocks at once (K = t b)
-            if (rc == HPLAI_OK && t > 0) rc = gemm(buf, 0, t, c, c, rb0 - c, b, s, kInBlock);
+            // the outer block take the t earlier inner blocks at once (K = t b);
+            // mid: the whole lower-right part of the outer block
+            if (rc == HPLAI_OK && mid) rc = gemm(buf, 0, split, c, c, rb0 - c, rb0 - c, s, kInBlock);
+            if (rc == HPLAI_OK && nsl > 0) rc = gemm(buf, base, nsl, c, c, rb0 - c, b, s, kInBlock);
             if (rc == HPLAI_OK) rc = lazy_init(c, c, rb0 - c, b, s);
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_DIAG_LU, 2.0 / 3.0 * bd * bd * bd, s, [&] {
@@ -438,7 +452,7 @@ struct LU {
             const int64_t ncols = rb0 - (c + b);
             if (rc) break;
             // block row t inside the outer block, likewise from the left (K = t b)
-            if (t > 0) rc = gemm(buf, 0, t, c, c + b, b, ncols, s, kInBlock);
+            if (nsl > 0) rc = gemm(buf, base, nsl, c, c + b, b, ncols, s, kInBlock);
             if (rc == HPLAI_OK) rc = lazy_init(c, c + b, b, ncols, s);
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_TRSM_R, static_cast<double>(ncols) * bd * bd, s, [&] {
@@ -467,11 +481,16 @@ struct LU {
         const int64_t Bw = static_cast<int64_t>(sw) * b;
         const int64_t col0 = c0 + Bw, ncols = n - col0;
         if (ncols <= 0) return rc;
+        const bool two = split > 0 && split < sw;
         for (int t = 0; t < sw && rc == HPLAI_OK; ++t) {
             const int64_t c = c0 + static_cast<int64_t>(t) * b;
             const double bd = static_cast<double>(b);
+            const int base = two && t >= split ? split : 0, nsl = t - base;
+            // with the split, row blocks split .. sw - 1 take blocks 0 .. split - 1 at once
+            if (two && t == split) rc = gemm(buf, 0, split, c, col0, Bw - static_cast<int64_t>(split) * b, ncols, s,
+                                             kBlockRow);
             // row block t takes the earlier row blocks at once (left-looking, K = t b)
-            if (t > 0) rc = gemm(buf, 0, t, c, col0, b, ncols, s, kBlockRow);
+            if (rc == HPLAI_OK && nsl > 0) rc = gemm(buf, base, nsl, c, col0, b, ncols, s, kBlockRow);
             if (rc == HPLAI_OK) rc = lazy_init(c, col0, b, ncols, s);   // block row right of the outer panel
             if (rc == HPLAI_OK)
                 rc = timed(HPLAI_PHASE_TRSM_R, static_cast<double>(ncols) * bd * bd, s, [&] {
@@ -780,6 +799,9 @@ int hplai_solve(hplai_ctx ctx, const hplai_params *p, hplai_result *res)
             lu.h_beta = -lu.h_beta;
         }
     }
+    HPLAI_CHECK(p->split >= 0 && (p->split == 0 || p->split < outer),
+                "hplai_solve: split %d must be 0 or below outer %d", p->split, outer);
+    lu.split = p->split;
     if (p->trail16) {
         // 16-bit trailing storage (the reference's -d double-decker lower deck):
         // partial sums of not yet factored blocks in FP16, scaled by

@@ -41,7 +41,7 @@ class Params(ctypes.Structure):
     _fields_ = [("n", _i64), ("b", _int), ("seed", _u64), ("fmt", _int), ("outer", _int),
                 ("lookahead", _int), ("lazy", _int), ("maxit", _int), ("verbose", _int),
                 ("keep_factors", _int), ("profile", _int), ("matrix", _int), ("cond", _dbl),
-                ("trail16", _int)]
+                ("trail16", _int), ("split", _int)]
 
 
 class Result(ctypes.Structure):
@@ -285,17 +285,20 @@ class Context:
 
     # ---- whole solve ----
     def solve(self, n, b, fmt="f16", outer=1, lookahead=True, lazy=False, seed=42, maxit=50,
-              verbose=False, keep_factors=False, profile=False, matrix="hplai", cond=0.0, trail16=False):
+              verbose=False, keep_factors=False, profile=False, matrix="hplai", cond=0.0, trail16=False,
+              split=0):
         """matrix: "hplai" (the reference's LCG matrix), "higham" (--highammat as
         the reference generates it) or "higham_kappa" (Fasi and Higham's signs,
         cond(A, inf) = cond exactly); cond 0 = 1000.  trail16 (with lazy): the
         partial sums of blocks not yet factored kept in FP16 (2: with the
-        compensated back buffer)."""
+        compensated back buffer).  split (0 < split < outer): each outer panel
+        is factored in two parts, the second taking the first split inner
+        blocks in one update."""
         p = Params(n=n, b=b, seed=seed, fmt=FMTS[fmt] if isinstance(fmt, str) else fmt, outer=outer,
                    lookahead=int(lookahead), lazy=1 if lazy else 0, maxit=maxit,
                    verbose=1 if verbose else 0, keep_factors=1 if keep_factors else 0,
                    profile=1 if profile else 0, matrix=MATRICES[matrix] if isinstance(matrix, str) else matrix,
-                   cond=cond, trail16=int(trail16))
+                   cond=cond, trail16=int(trail16), split=int(split))
         r = Result()
         rc = lib().hplai_solve(self.handle, ctypes.byref(p), ctypes.byref(r))
         if rc not in (OK, ENOCONV):

@@ -264,6 +264,11 @@ typedef struct hplai_params {
                             panel.hpp:45-83, panel_trf.hpp:343-345); b % 128 == 0.
                             2: also the compensated back buffer (--back_buffer,
                             back_buffer.hpp:81-170), a second FP16 n x n array           */
+    int      split;      /* 0, or h with 0 < h < outer: each outer panel is factored in two
+                            parts; after its first h inner blocks the rest of the outer
+                            block and of its block row take them in one update (K = h b),
+                            and the later inner blocks look left only as far as block h.
+                            The trailing update keeps K = outer b.                       */
 } hplai_params;
 
 #define HPLAI_MATRIX_HPLAI        0   /* the reference's LCG matrix with diagonal row sums */

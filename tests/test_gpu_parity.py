@@ -405,8 +405,10 @@ def test_solve_c1_matches_oracle(ctx, oracle):
     assert np.max(np.abs(x - orc["x"])) / np.max(np.abs(orc["x"])) < 1e-10
 
 
-@pytest.mark.parametrize("outer,lookahead,lazy", [(4, True, False), (5, True, True), (20, False, False)])
-def test_solve_outer_blocked_matches_oracle(ctx, oracle, outer, lookahead, lazy):
+@pytest.mark.parametrize("outer,lookahead,lazy,split", [(4, True, False, 0), (5, True, True, 0),
+                                                        (20, False, False, 0), (8, True, False, 3),
+                                                        (6, 2, True, 2)])
+def test_solve_outer_blocked_matches_oracle(ctx, oracle, outer, lookahead, lazy, split):
     """The outer-blocked schedule (left-looking inside an outer block of
     `outer` panels, one K = outer*b trailing update) quantizes every L and U
     panel at the same point of the elimination as the reference's
@@ -415,14 +417,14 @@ def test_solve_outer_blocked_matches_oracle(ctx, oracle, outer, lookahead, lazy)
     is not a multiple of 128, so no 16-bit storage) adds a_ij to -sum L U
     instead of the other way round: the same arithmetic up to order."""
     n, b = 1200, 60
-    res = ctx.solve(n, b, fmt="f16", outer=outer, lookahead=lookahead, lazy=lazy, keep_factors=True)
+    res = ctx.solve(n, b, fmt="f16", outer=outer, lookahead=lookahead, lazy=lazy, keep_factors=True, split=split)
     lu = ctx.solution_factors(n)
     x = ctx.solution_x(n)
     ctx.release()
     ref, d = oracle.lu(oracle.LU_PRECOMP, n, b, SEED)
     orc = oracle.ir(ref, n, d)
     assert res["converged"] and res["harness"] < 16.0 and res["ir_steps"] == orc["steps"]
-    _check_factors(lu, ref, "f16", "c1 outer %d lazy %d" % (outer, lazy))
+    _check_factors(lu, ref, "f16", "c1 outer %d lazy %d split %d" % (outer, lazy, split))
     assert np.max(np.abs(x - orc["x"])) / np.max(np.abs(orc["x"])) < 1e-10
 
 
@@ -455,32 +457,38 @@ def _block_residuals(ctx_read, oracle, n, b, fmt, blocks):
     return worst
 
 
-@pytest.mark.parametrize("fmt,outer,mode,lookahead", [("f16", 2, "LU_PRECOMP", 1),
-                                                      ("e4m3", 1, "LU_FP8", 1),
-                                                      # 4 and 2 outer steps: K = outer*b trailing
-                                                      # updates on the streamed kernel, per-slice
-                                                      # exponents of the FP32-C trailing path
-                                                      ("e4m3", 4, "LU_FP8", 1),
-                                                      ("e4m3", 8, "LU_FP8", 2),
-                                                      # row look-ahead: block row k+1 on the side stream
-                                                      ("f16", 2, "LU_PRECOMP", 2),
-                                                      ("e4m3", 1, "LU_FP8", 2)])
-def test_solve_packed_operands_match_oracle(ctx, oracle, fmt, outer, mode, lookahead):
+@pytest.mark.parametrize("fmt,outer,mode,lookahead,split", [("f16", 2, "LU_PRECOMP", 1, 0),
+                                                            ("e4m3", 1, "LU_FP8", 1, 0),
+                                                            # 4 and 2 outer steps: K = outer*b trailing
+                                                            # updates on the streamed kernel, per-slice
+                                                            # exponents of the FP32-C trailing path
+                                                            ("e4m3", 4, "LU_FP8", 1, 0),
+                                                            ("e4m3", 8, "LU_FP8", 2, 0),
+                                                            # row look-ahead: block row k+1 on the side stream
+                                                            ("f16", 2, "LU_PRECOMP", 2, 0),
+                                                            ("e4m3", 1, "LU_FP8", 2, 0),
+                                                            # outer panels factored in two parts
+                                                            ("e4m3", 8, "LU_FP8", 1, 4),
+                                                            ("e4m3", 8, "LU_FP8", 2, 3),
+                                                            ("f16", 4, "LU_PRECOMP", 1, 1)])
+def test_solve_packed_operands_match_oracle(ctx, oracle, fmt, outer, mode, lookahead, split):
     """b = 512: the quantized panels are stored panel-packed (256-row panels of
     64-byte K stages) and read by the streamed 256 x 256 Schur kernel
     (n = 8192 has >= 2 tiles per CU in the first updates).  The factors must
     match the oracle's LU of the same format part by part, and sampled
     blocks of the factors must reproduce the matrix from the quantized
     operands (parity.block_residual), which localises a misplaced chunk,
-    stage or exponent to the blocks it touched."""
+    stage or exponent to the blocks it touched.  With a split the panel's
+    second part takes the first `split` inner blocks in one update: the same
+    products summed in another order, so the same bounds hold."""
     n, b = 8192, 512
-    res = ctx.solve(n, b, fmt=fmt, outer=outer, lookahead=lookahead, keep_factors=True)
+    res = ctx.solve(n, b, fmt=fmt, outer=outer, lookahead=lookahead, keep_factors=True, split=split)
     lu = ctx.solution_factors(n)
     ctx.release()
     ref, d = oracle.lu(getattr(oracle, mode), n, b, SEED)
     assert res["converged"] and res["harness"] < 16.0
     assert res["ir_steps"] == _oracle_steps(oracle, ref, n, d), res["ir_steps"]
-    _check_factors(lu, ref, fmt, "n8192 %s outer %d la %d" % (fmt, outer, lookahead))
+    _check_factors(lu, ref, fmt, "n8192 %s outer %d la %d split %d" % (fmt, outer, lookahead, split))
     read = lambda i0, j0, m, k: lu[j0:j0 + k, i0:i0 + m]
     nb = n // b
     worst = _block_residuals(read, oracle, n, b, fmt, [(0, 0), (1, 0), (outer, outer), (outer + 1, outer),
