@@ -79,7 +79,8 @@ def run_form(ctx, args, fmt, outer):
         classes[name] = {"launches": nl // len(results), "ms_per_solve": ms / len(results),
                          "tflops": fl / (ms * 1e-3) * 1e-12 if ms > 0 else None}
     tot["schur_classes"] = classes
-    return dict(fmt=fmt, outer=outer, split=_split(args, outer), time_s=t, wall_s=wall, tflops=hpl_flops(args.n) / t * 1e-12,
+    return dict(fmt=fmt, outer=outer, split=_split(args, outer), time_s=t, wall_s=wall,
+                tflops=hpl_flops(args.n) / t * 1e-12,
                 time_lu_s=res["time_lu"], time_ir_s=res["time_ir"], time_gen_s=res["time_gen"],
                 ir_steps=res["ir_steps"], harness=res["harness"], residual=res["residual"],
                 schur_tflops=(tot["schur_flops"] / (tot["schur_ms"] * 1e-3) * 1e-12)
@@ -92,7 +93,9 @@ def roofline(p, fmt, n):
     duration, both from the library's per-launch HIP events on the launch
     stream.  `traffic` is the measured HBM bytes per launch of that kernel,
     read from the committed rocprofv3 counter summary of this workload when
-    one exists (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel)."""
+    one exists (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel);
+    `counters` the shader clock, matrix-core occupancy and L2 hit rate the
+    committed counter summary measured for it."""
     peak = PEAK_TFLOPS[fmt]
     nl = p["schur_large_launches"]
     if not nl or p["schur_large_ms"] <= 0:
@@ -107,8 +110,22 @@ def roofline(p, fmt, n):
         for line in open(summ):
             if line.startswith("hbm_bytes_per_launch"):
                 traffic = float(line.split()[1])
+    # clock and matrix-core occupancy of the same kernel from the committed
+    # counter summary of this workload (the first kernel block of the file)
+    counters = None
+    csumm = os.path.join(ROOT, "profiles", "schur_counters_%s_n%d.txt" % (fmt, n))
+    if os.path.exists(csumm):
+        vals = {}
+        for line in open(csumm):
+            f = line.split()
+            if len(f) == 2 and f[0] in ("sclk_mhz", "mfma_busy_pct", "l2_hit_pct") and f[0] not in vals:
+                vals[f[0]] = float(f[1])
+        if vals:
+            counters = dict(vals, source=os.path.relpath(csumm, ROOT))
+            if "sclk_mhz" in vals:
+                counters["peak_at_sclk"] = peak * vals["sclk_mhz"] / 2400.0
     return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
-            "traffic": traffic,
+            "traffic": traffic, "counters": counters,
             "kernel": "schur_dma_kernel<%s> (256x256 streamed Schur update)" % fmt,
             "launches": nl, "flops_per_launch": flops_per, "ms_per_launch": ms_per,
             "algorithmic_bytes_per_launch": p["schur_large_bytes"] / nl,
