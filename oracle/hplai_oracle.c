@@ -11,6 +11,7 @@
  */
 #include "hplai_oracle.h"
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
@@ -445,6 +446,13 @@ static void quant_tiles(int mode, int64_t m, int64_t k, int b, const float *src,
         }
 }
 
+/* With HPLAI_ORACLE_PROGRESS set, the long oracle runs (the LU and the
+ * refinement at full size take minutes) report each step on stderr. */
+static int orc_progress(void)
+{
+    return getenv("HPLAI_ORACLE_PROGRESS") != NULL;
+}
+
 int orc_lu(int mode, int64_t n, int b, uint64_t seed, const float *dhigh,
            float *A, int64_t lda, int nthreads)
 {
@@ -491,6 +499,10 @@ int orc_lu(int mode, int64_t n, int b, uint64_t seed, const float *dhigh,
                 }
             }
             (void)jn;
+        }
+        if (orc_progress() && k % 8 == 0) {
+            fprintf(stderr, "orc_lu: block %lld of %lld\n", (long long)k, (long long)nb);
+            fflush(stderr);
         }
         /* getrf_nopiv on the diagonal block (panel_trf.hpp:771) */
         orc_getrf_nopiv_f32(b, Akk, lda);
@@ -721,6 +733,10 @@ int orc_ir(int64_t n, uint64_t seed, const float *lu, int64_t lda, const double 
         double h = normr / (norma * normx + normb) * 1. / ((double)n * DBL_EPSILON / 2);
         if (trace_resid) trace_resid[iter] = normr;
         if (trace_harness) trace_harness[iter] = h;
+        if (orc_progress()) {
+            fprintf(stderr, "orc_ir: step %d harness %g\n", iter, h);
+            fflush(stderr);
+        }
         if (h < 16.) {
             *final_resid = normr;
             *final_harness = h;

@@ -81,6 +81,33 @@ def checksums(lu, n=None):
                      np.sum(np.abs(lo), dtype=np.float64)])
 
 
+def factor_errors_chunked(lu, ref, n=None, step=2048):
+    """factor_errors over column chunks of `step` (the same numbers, with
+    memory bounded by a few chunks instead of several n x n copies)."""
+    n = lu.shape[0] if n is None else n
+    dmax = {"l": 0.0, "u": 0.0, "d": 0.0}
+    smax = {"l": 0.0, "u": 0.0, "d": 0.0}
+    sa = np.zeros(3)
+    sb = np.zeros(3)
+    for j0 in range(0, n, step):
+        j1 = min(n, j0 + step)
+        x = np.asarray(lu[j0:j1, :n], dtype=np.float64)    # columns j0 .. j1, all rows
+        y = np.asarray(ref[j0:j1, :n], dtype=np.float64)
+        rows = np.arange(n)[None, :]
+        cols = np.arange(j0, j1)[:, None]
+        for key, mask in (("l", rows > cols), ("u", rows < cols), ("d", rows == cols)):
+            xm, ym = np.where(mask, x, 0.0), np.where(mask, y, 0.0)
+            dmax[key] = max(dmax[key], float(np.max(np.abs(xm - ym))))
+            smax[key] = max(smax[key], float(np.max(np.abs(ym))))
+            k = {"d": 0, "u": 1, "l": 2}[key]
+            sa[k] += float(np.sum(np.abs(xm)))
+            sb[k] += float(np.sum(np.abs(ym)))
+    out = {k: (dmax[k] / smax[k] if smax[k] > 0 else (0.0 if dmax[k] == 0 else np.inf)) for k in dmax}
+    rel = float(np.max(np.abs(sa - sb) / np.abs(sb)))
+    out["sums"] = rel if np.isfinite(rel) else np.inf
+    return out
+
+
 def within(err, bounds=None):
     """True when every part is inside its bound."""
     bounds = SAME_FORMAT if bounds is None else bounds

@@ -151,3 +151,19 @@ def test_tile_exponents_vary_and_are_pinned(oracle):
     assert P.within(err, P.bounds("e4m3")), P.describe(err)
     bad = M.to_colmajor(M.lu(As.T.copy(), B, "e4m3", mutate="tile_exp"))
     assert not P.within(P.factor_errors(bad, ref), P.bounds("e4m3"))
+
+
+def test_chunked_metric_equals_the_whole_array_metric(oracle):
+    """parity.factor_errors_chunked (for n too large for whole-array copies)
+    gives the numbers of parity.factor_errors, here on an oracle LU with a
+    perturbed L and a perturbed U entry."""
+    n, b = 1200, 60
+    ref, _ = oracle.lu(oracle.LU_PRECOMP, n, b)
+    lu = ref.copy()
+    lu[5, 100] += 1e-3      # strict L (row 100 > column 5)
+    lu[700, 3] *= 1.5       # strict U (row 3 < column 700)
+    whole = P.factor_errors(lu, ref)
+    for step in (256, 1000, 2048):
+        chunked = P.factor_errors_chunked(lu, ref, step=step)
+        for k in whole:
+            assert abs(whole[k] - chunked[k]) <= 1e-12 * max(1.0, whole[k]), (k, step)
