@@ -439,6 +439,9 @@ constexpr int kDmaStageBytes = 64;                                  // K bytes p
 constexpr int kDmaStages = 4;
 constexpr int kDmaStage = 2 * kDmaTile * kDmaStageBytes;           // 32 KB
 constexpr int kDmaLoadsPerStage = 2 * kDmaTile * kDmaStageBytes / (kThreads * 16);   // 8 per thread
+// e4m3 exponents of one launch staged in LDS (slices per launch; longer
+// updates take the 128 x 128 kernel)
+constexpr int kDmaMaxSlices = 2048;
 
 // EPI: 0 = element-wise C access (unaligned C), 1 = float4 per accumulator
 // row block with ragged edges, 2 = whole tiles only (M, N multiples of 256),
@@ -455,7 +458,12 @@ __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
     static_assert(kDmaLoadsPerStage == 8, "four Lq and four Uq loads per thread per stage");
     using F = SchurFmt<FMT>;
     constexpr int BM = kDmaTile, BN = kDmaTile, SB = kDmaStageBytes;
-    __shared__ __attribute__((aligned(1024))) uint8_t S[kDmaStages * kDmaStage];
+    // the operand stages, then (e4m3) the exponents of the launch's slices;
+    // one array, so that the compiler sees a single LDS object written by the
+    // direct loads
+    constexpr int kExpBytes = FMT == HPLAI_FMT_E4M3 ? 2 * kDmaMaxSlices * 4 : 0;
+    __shared__ __attribute__((aligned(1024))) uint8_t S[kDmaStages * kDmaStage + kExpBytes];
+    int32_t *const SE = reinterpret_cast<int32_t *>(S + kDmaStages * kDmaStage);
 
     const int nwg = tiles_m * tiles_n;
     const int logical = xcd_remap(static_cast<int>(blockIdx.x), nwg);
@@ -515,6 +523,20 @@ __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
 
     const int row_tile = tb > 0 ? static_cast<int>(m0 / tb) : 0;
     const int col_tile = tb > 0 ? static_cast<int>(n0 / tb) : 0;
+    // the tile's exponents of every slice go to LDS before the operand
+    // stream starts: read in the loop they are ordered with the operand
+    // reads, where a scalar load would make every wait on those reads also
+    // wait for it (and a scalar-cache miss behind the operand stream costs
+    // more than a stage).  (Staged after the prologue's operand loads
+    // instead, the LDS writes wait for all four stages, not just the first:
+    // measured slower.)
+    if constexpr (FMT == HPLAI_FMT_E4M3) {
+        for (int t = tid; t < nslices; t += kThreads) {
+            SE[t] = eL[t * ldel + row_tile];
+            SE[kDmaMaxSlices + t] = eU[t * ldeu + col_tile];
+        }
+        __syncthreads();
+    }
 
     // operand registers of one stage; two sets alternate so the reads of
     // stage ks + 1 overlap the second half of stage ks's MFMAs
@@ -562,16 +584,15 @@ __global__ __launch_bounds__(kThreads) void schur_dma_kernel(
             }
         }
     };
-    // E4M3 scales: the exponents of the stage after next are fetched one step
-    // early and consumed right after the workgroup barrier, where no LDS read
-    // is outstanding (a scalar-load wait would otherwise also wait for them)
+    // E4M3 scales: the exponents of the stage after next are read from LDS one
+    // step early and consumed right after the workgroup barrier
     int pe_l = 0, pe_u = 0;
     auto fetch_exps = [&]() {
         if constexpr (FMT == HPLAI_FMT_E4M3) {
             // fetches past the last stage (never used) keep to the last slice
             const int sl = rd_slice < nslices ? rd_slice : nslices - 1;
-            pe_l = eL[sl * ldel + row_tile];
-            pe_u = eU[sl * ldeu + col_tile];
+            pe_l = SE[sl];
+            pe_u = SE[kDmaMaxSlices + sl];
         }
         if (++rd_in_slice == stages_per_slice) {
             rd_in_slice = 0;
@@ -771,7 +792,8 @@ static int schur_dispatch(int64_t m, int64_t n, int nslices, int kslice, const v
 {
     if constexpr (FMT != HPLAI_FMT_F16_TILE) {
         // the FP16 trailing storage takes the streamed kernel only for whole 256 x 256 tiles
-        if (schur_large_tiles(FMT, m, n, tb, num_cu) && (!c16 || (m % kDmaTile == 0 && n % kDmaTile == 0)))
+        if (schur_large_tiles(FMT, m, n, tb, num_cu) && (!c16 || (m % kDmaTile == 0 && n % kDmaTile == 0)) &&
+            (FMT != HPLAI_FMT_E4M3 || nslices <= kDmaMaxSlices))
             return schur_dma_launch<FMT, PACKED>(m, n, nslices, kslice, Lq, ldl, eL, ldel, Uq, ldu, eU, ldeu,
                                          tb, C, ldc, s, c16, cscale);
     }
