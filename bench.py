@@ -299,6 +299,65 @@ def bench_autograd(args):
 DBL_EPS = 2.220446049250313e-16
 
 
+def bench_schur(args):
+    """Kernel form: the Schur update alone, C -= 2^-(eL+eU) Lq Uq with C in
+    FP32 (read and written), m = n = args.schur_m, K = 8192, through the C
+    ABI (row-layout operands), beside hipBLASLt's GEMM of the same shape
+    through torch (C = A B^T, C not read: FP8 with FP32 and with BF16 output,
+    FP16 with FP16 output) as a reference point for the kernel's rate."""
+    import torch
+    import hplai
+    m = n = args.schur_m
+    K, tb = 8192, 512
+    fl = 2.0 * m * n * K
+    ctx = hplai.Context(0)
+    g = torch.Generator(device="cuda").manual_seed(1)
+    A8 = torch.randn(m, K, device="cuda", generator=g).to(torch.float8_e4m3fn)
+    B8 = torch.randn(n, K, device="cuda", generator=g).to(torch.float8_e4m3fn)
+    eL = torch.zeros(K // tb, m // tb, dtype=torch.int32, device="cuda")
+    eU = torch.zeros(K // tb, n // tb, dtype=torch.int32, device="cuda")
+    C = torch.zeros(n, m, dtype=torch.float32, device="cuda")
+    one = torch.ones((), device="cuda")
+    reps = max(args.steps, 3)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        ts.sort()
+        return {"ms_min": ts[0], "ms_median": ts[len(ts) // 2], "tflops": fl / (ts[len(ts) // 2] * 1e-3) / 1e12}
+
+    res = {}
+    L8, U8 = A8.view(torch.uint8), B8.view(torch.uint8)
+    res["schur_e4m3"] = timed(lambda: ctx.schur_update(2, m, n, K // tb, tb, L8, K, eL, m // tb, U8, K, eU,
+                                                       n // tb, tb, C, m))
+    res["hipblaslt_fp8_out_f32"] = timed(lambda: torch._scaled_mm(A8, B8.t(), scale_a=one, scale_b=one,
+                                                                  out_dtype=torch.float32))
+    res["hipblaslt_fp8_out_bf16"] = timed(lambda: torch._scaled_mm(A8, B8.t(), scale_a=one, scale_b=one,
+                                                                   out_dtype=torch.bfloat16))
+    del A8, B8, L8, U8
+    A16 = torch.randn(m, K, device="cuda", generator=g).half()
+    B16 = torch.randn(n, K, device="cuda", generator=g).half()
+    res["schur_f16"] = timed(lambda: ctx.schur_update(0, m, n, K // tb, tb, A16, K, eL, m // tb, B16, K, eU,
+                                                      n // tb, tb, C, m))
+    res["hipblaslt_f16_out_f16"] = timed(lambda: torch.mm(A16, B16.t()))
+    ctx.close()
+    e4 = res["schur_e4m3"]
+    print(json.dumps({
+        "metric": "Schur update alone, m=n=%d, K=8192: TFLOP/s (median of %d), 1 GPU" % (m, reps),
+        "value": e4["tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": reps, "warmup": 1,
+        "ms_per_step": e4["ms_median"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "e4m3", "data": "synthetic (N(0,1) operands, zero exponents)",
+        "config": {"workload": "C(%d x %d, FP32) -= L U, K = 8192" % (m, n)}, "kernels": res}))
+
+
 def backward_error(harness, n):
     """Normwise backward error ||b - A x|| / (||A|| ||x|| + ||b||) (inf-norms)
     from the HPL-AI harness, which is that ratio divided by n eps / 2."""
@@ -337,8 +396,10 @@ def fp64_comparison(ctx, args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--form", default="solve", choices=["solve", "batch", "autograd"],
-                    help="solve: the metric's workload (default); batch / autograd: the other forms")
+    ap.add_argument("--form", default="solve", choices=["solve", "batch", "autograd", "schur"],
+                    help="solve: the metric's workload (default); batch / autograd: the other forms; "
+                         "schur: the Schur kernel alone beside hipBLASLt")
+    ap.add_argument("--schur-m", type=int, default=32768)
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
@@ -380,6 +441,8 @@ def main():
         return bench_batch(args)
     if args.form == "autograd":
         return bench_autograd(args)
+    if args.form == "schur":
+        return bench_schur(args)
     if args.copy_check_bytes > 0:
         src = torch.ones(args.copy_check_bytes // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
