@@ -1,7 +1,7 @@
 /*
  * seam_example.c — the reference's Schur seam bound to libhplai_gfx950.so
- * exactly as INTEGRATION.md shows a maintainer doing it, in plain C built
- * with gcc (no hipcc, no C++).
+ * the way a maintainer of the reference's host code would bind it, in plain
+ * C built with gcc (no hipcc, no C++).
  *
  * The reference's overload (schur_updator.hpp:28-31)
  *     inline void gemmschur(int n, int m, int k, fp16 const *a, int lda,

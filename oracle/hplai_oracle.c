@@ -238,7 +238,7 @@ float orc_e4m3_to_f32(uint8_t q)
 /* Per-tile power-of-two exponent: the largest e with amax * 2^e <= fmt_max,
  * where fmt_max = mr * 2^er (448 = 1.75*2^8 for e4m3; 65504 = (2-2^-10)*2^15
  * for binary16).  Exact integer arithmetic on the FP32 bits of amax.
- * (North-star generalisation of the reference's scalea, panel_trf.hpp:717.) */
+ * (The e4m3 form generalises the reference's scalea, panel_trf.hpp:717.) */
 static int tile_exp(float amax, uint32_t mr_frac_bits, int er)
 {
     if (!(amax > 0.0f)) return 0;
@@ -529,7 +529,7 @@ int orc_lu(int mode, int64_t n, int b, uint64_t seed, const float *dhigh,
                     U16[j * b + i] = orc_f32_to_f16(U[j * lda + i]);
             orc_schur_f16acc(m, m, b, L16, m, U16, b, P + (kk + b) * n + kk + b, n);
         } else {
-            /* FP8 e4m3 / per-tile power-of-two (north-star form, see F3) */
+            /* FP8 e4m3 / per-tile power-of-two scales (generalises scalea, panel_trf.hpp:717-719) */
             quant_tiles(mode, m, b, b, L, lda, Lq, m);
             quant_tiles(mode, b, m, b, U, lda, Uq, b);
             orc_schur_f32acc(m, m, b, Lq, m, Uq, b, C, lda);

@@ -629,7 +629,7 @@ def test_driver_delayed_trailing_storage(tmp_path, extra, trailing):
 
 # ------------------------------------------------------------------ the reference's gemmschur seam
 def _seam_run(tmp_path, a, lda, b, ldb, c, ldc, n, m, k):
-    """Run bin/seam_example (plain C, gcc, bound as INTEGRATION.md shows)."""
+    """Run bin/seam_example (plain C, gcc, bound the way the reference's host code would)."""
     import os
     import subprocess
     import hplai
@@ -673,7 +673,7 @@ def test_gemmschur_seam_reference_fixture(tmp_path):
                                                (300, 77, 1, 300, 1, 301), (129, 255, 65, 129, 70, 130)])
 def test_gemmschur_seam_vs_oracle(tmp_path, oracle, n, m, k, lda, ldb, ldc):
     """C1 shapes (k = b = 60, not a multiple of the 64-value stage) and ragged
-    ones, bound from C exactly as INTEGRATION.md shows, against the oracle's
+    ones, bound from plain C the way the reference's host code would, against the oracle's
     shgemm restatement (bit-exact with the reference's shgemm_opt,
     test_oracle.py) within the FP32-summation-order bound."""
     rng = np.random.default_rng(n + m + k)
