@@ -367,10 +367,10 @@ __global__ __launch_bounds__(kGetrfThreads) void getrf_mfma_kernel(int n, float 
         }
         __syncthreads();
         // 5. trailing update A22 -= L21 U12 (hr x wr, both multiples of 32):
-        //    two tiles per wave at a time; the C of the wave's next two tiles
-        //    loads while these two are computed
+        //    one tile per wave at a time; the C of the wave's next tile loads
+        //    while this one is computed (two at a time: 5% slower, four: spills)
         const int ntile = TR * TC;
-        constexpr int TPW = 2, STEP = TPW * (NT / 64);
+        constexpr int TPW = 1, STEP = TPW * (NT / 64);
         auto load_c = [&](int t0, float4 (&v)[TPW][4]) {
 #pragma unroll
             for (int u = 0; u < TPW; ++u) {
