@@ -358,6 +358,50 @@ def bench_schur(args):
         "config": {"workload": "C(%d x %d, FP32) -= L U, K = 8192" % (m, n)}, "kernels": res}))
 
 
+def bench_panel(args):
+    """Panel form: the kernels of one panel step of the LU, alone (b = 512):
+    the diagonal-block LU, and the two panel TRSMs over m = n - b rows /
+    columns (each C-ABI TRSM call also inverts the diagonal block's
+    triangles).  The tail of the LU is a chain of these, one panel after the
+    other; a kernel trace of this form gives each kernel's own duration."""
+    import torch
+    import hplai
+    b, m = args.b, args.n - args.b
+    ctx = hplai.Context(0)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    A0 = (torch.rand(b, b, device="cuda", generator=g) - 0.5) + torch.eye(b, device="cuda") * b
+    Ukk = A0.triu().t().contiguous()                      # column-major upper triangle
+    L = torch.rand(b, m, device="cuda", generator=g) - 0.5   # column-major m x b
+    U = torch.rand(m, b, device="cuda", generator=g) - 0.5   # column-major b x m
+    reps = max(args.steps, 5)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        ts.sort()
+        return {"ms_min": ts[0], "ms_median": ts[len(ts) // 2]}
+
+    A = A0.clone()   # factored again in place by every call (the factors stay diagonally dominant)
+    res = {"getrf_b512": timed(lambda: ctx.getrf(b, A, b)),
+           "trsm_l_m%d" % m: timed(lambda: ctx.trsm_lpanel(m, b, Ukk, b, L, m)),
+           "trsm_u_n%d" % m: timed(lambda: ctx.trsm_upanel(b, m, Ukk, b, U, b))}
+    ctx.close()
+    print(json.dumps({
+        "metric": "panel-step kernels alone, b=%d, m=%d: ms per call (median of %d), 1 GPU" % (b, m, reps),
+        "value": res["getrf_b512"]["ms_median"], "unit": "ms", "n_gpus": 1, "steps": reps, "warmup": 1,
+        "ms_per_step": res["getrf_b512"]["ms_median"], "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (uniform operands, dominant diagonal)",
+        "config": {"workload": "getrf b=%d; TRSM L m x b and U b x m, m=%d" % (b, m)}, "kernels": res}))
+
+
 def backward_error(harness, n):
     """Normwise backward error ||b - A x|| / (||A|| ||x|| + ||b||) (inf-norms)
     from the HPL-AI harness, which is that ratio divided by n eps / 2."""
@@ -396,9 +440,9 @@ def fp64_comparison(ctx, args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--form", default="solve", choices=["solve", "batch", "autograd", "schur"],
+    ap.add_argument("--form", default="solve", choices=["solve", "batch", "autograd", "schur", "panel"],
                     help="solve: the metric's workload (default); batch / autograd: the other forms; "
-                         "schur: the Schur kernel alone beside hipBLASLt")
+                         "schur: the Schur kernel alone beside hipBLASLt; panel: one panel step's kernels alone")
     ap.add_argument("--schur-m", type=int, default=32768)
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--gpus", type=int, default=1)
@@ -443,6 +487,8 @@ def main():
         return bench_autograd(args)
     if args.form == "schur":
         return bench_schur(args)
+    if args.form == "panel":
+        return bench_panel(args)
     if args.copy_check_bytes > 0:
         src = torch.ones(args.copy_check_bytes // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
