@@ -305,22 +305,41 @@ __global__ __launch_bounds__(256) void tri_inv_kernel(int b, const float *__rest
     }
     __syncthreads();
     const int steps = upper ? j : nb - 1 - j;
+    // the L (U) blocks of a step do not depend on the step before it: each
+    // wave loads its (at most four, nb <= 16) blocks of step st + 1 while step
+    // st is reduced and multiplied by the sub-block inverse
+    auto load_blocks = [&](int st, float (&av)[4][16]) {
+        const int i = upper ? j - st : j + st;
+        const int klo = upper ? i + 1 : j, khi = upper ? j : i - 1;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = klo + wave + 4 * u;
+            if (k <= khi) {
+                // A = L(i, k) or U(i, k): element (32 i + l32, 32 k + 2 kk + lh)
+                const float *src = LU + static_cast<int64_t>(32 * k + lh) * lda + 32 * i + l32;
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk) av[u][kk] = src[static_cast<int64_t>(2 * kk) * lda];
+            }
+        }
+    };
+    float avc[4][16], avn[4][16];
+    if (steps >= 1) load_blocks(1, avc);
     for (int st = 1; st <= steps; ++st) {
         const int i = upper ? j - st : j + st;
         const int klo = upper ? i + 1 : j, khi = upper ? j : i - 1;
+        if (st < steps) load_blocks(st + 1, avn);
         floatx16t_t acc;
 #pragma unroll
         for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
-        for (int k = klo + wave; k <= khi; k += 4) {
-            // A = L(i, k) or U(i, k): element (32 i + l32, 32 k + 2 kk + lh)
-            const float *src = LU + static_cast<int64_t>(32 * k + lh) * lda + 32 * i + l32;
-            float av[16];
 #pragma unroll
-            for (int kk = 0; kk < 16; ++kk) av[kk] = src[static_cast<int64_t>(2 * kk) * lda];
+        for (int u = 0; u < 4; ++u) {
+            const int k = klo + wave + 4 * u;
+            if (k <= khi) {
 #pragma unroll
-            for (int kk = 0; kk < 16; ++kk)
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kk], W[(32 * k + 2 * kk + lh) * kXS + l32], acc, 0,
-                                                           0, 0);
+                for (int kk = 0; kk < 16; ++kk)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(avc[u][kk], W[(32 * k + 2 * kk + lh) * kXS + l32],
+                                                               acc, 0, 0, 0);
+            }
         }
 #pragma unroll
         for (int e = 0; e < 16; ++e) P[wave * 1024 + lane * 16 + e] = acc[e];
@@ -355,6 +374,10 @@ __global__ __launch_bounds__(256) void tri_inv_kernel(int b, const float *__rest
                 }
         }
         __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk) avc[u][kk] = avn[u][kk];
     }
 }
 
