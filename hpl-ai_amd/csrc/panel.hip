@@ -273,53 +273,52 @@ __global__ __launch_bounds__(kGetrfThreads) void getrf_mfma_kernel(int n, float 
         }
         __syncthreads();
         if (wr <= 0) break;
-        // 3. L11^-1 (wave 0) and U11^-1 (wave 1), Gauss-Jordan on rows held one per lane
-        if (wave == 0) {
-            const int i = l32;
-            float a[W], x[W];
+        // 3. L11^-1 (waves 0 and 2) and U11^-1 (waves 1 and 3), Gauss-Jordan on
+        //    rows held one per lane; the columns of an inverse are independent,
+        //    so each of the two waves takes every second one (j = 2 jj + par)
+        if (wave < 4) {
+            const int i = l32, par = wave >> 1;
+            float a[W], x[W / 2];
 #pragma unroll
-            for (int j = 0; j < W; ++j) {
-                a[j] = Lc[j * kLcS + i];
-                x[j] = j == i ? 1.0f : 0.0f;
-            }
-            // rows i > c: X(i, :) -= L(i, c) X(c, :), X(c, :) final at step c
+            for (int j = 0; j < W; ++j) a[j] = Lc[j * kLcS + i];
 #pragma unroll
-            for (int c = 0; c < W; ++c) {
-                const bool below = i > c;
+            for (int jj = 0; jj < W / 2; ++jj) x[jj] = 2 * jj + par == i ? 1.0f : 0.0f;
+            if ((wave & 1) == 0) {
+                // rows i > c: X(i, :) -= L(i, c) X(c, :), X(c, :) final at step c
 #pragma unroll
-                for (int j = 0; j <= c; ++j) {
-                    const float xcj = readlane_f(x[j], c);
-                    x[j] = below ? fmaf(-a[c], xcj, x[j]) : x[j];
+                for (int c = 0; c < W; ++c) {
+                    const bool below = i > c;
+#pragma unroll
+                    for (int jj = 0; jj < W / 2; ++jj)
+                        if (2 * jj + par <= c) {
+                            const float xcj = readlane_f(x[jj], c);
+                            x[jj] = below ? fmaf(-a[c], xcj, x[jj]) : x[jj];
+                        }
                 }
-            }
-            if (lh == 0)
+                if (lh == 0)
 #pragma unroll
-                for (int j = 0; j < W; ++j) Li[i * 33 + j] = x[j];
-        } else if (wave == 1) {
-            const int i = l32;
-            float a[W], y[W];
+                    for (int jj = 0; jj < W / 2; ++jj) Li[i * 33 + 2 * jj + par] = x[jj];
+            } else {
+                // from the bottom: Y(c, :) /= U(c, c), then rows i < c: Y(i, :) -= U(i, c) Y(c, :)
 #pragma unroll
-            for (int j = 0; j < W; ++j) {
-                a[j] = Lc[j * kLcS + i];
-                y[j] = j == i ? 1.0f : 0.0f;
-            }
-            // from the bottom: Y(c, :) /= U(c, c), then rows i < c: Y(i, :) -= U(i, c) Y(c, :)
+                for (int c = W - 1; c >= 0; --c) {
+                    const float rd = 1.0f / readlane_f(a[c], c);
+                    if (i == c)
 #pragma unroll
-            for (int c = W - 1; c >= 0; --c) {
-                const float rd = 1.0f / readlane_f(a[c], c);
-                if (i == c)
+                        for (int jj = 0; jj < W / 2; ++jj)
+                            if (2 * jj + par >= c) x[jj] *= rd;
+                    const bool above = i < c;
 #pragma unroll
-                    for (int j = c; j < W; ++j) y[j] *= rd;
-                const bool above = i < c;
-#pragma unroll
-                for (int j = c; j < W; ++j) {
-                    const float ycj = readlane_f(y[j], c);
-                    y[j] = above ? fmaf(-a[c], ycj, y[j]) : y[j];
+                    for (int jj = 0; jj < W / 2; ++jj)
+                        if (2 * jj + par >= c) {
+                            const float ycj = readlane_f(x[jj], c);
+                            x[jj] = above ? fmaf(-a[c], ycj, x[jj]) : x[jj];
+                        }
                 }
-            }
-            if (lh == 0)
+                if (lh == 0)
 #pragma unroll
-                for (int j = 0; j < W; ++j) Ui[i * 33 + j] = y[j];
+                    for (int jj = 0; jj < W / 2; ++jj) Ui[i * 33 + 2 * jj + par] = x[jj];
+            }
         }
         __syncthreads();
         // 4. L21 := A21 U11^-1 (TR tiles) and U12 := L11^-1 A12 (TC tiles)
